@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3D Yee FDTD time-stepping path on one MI355X.
+
+Workload (default): 512^3 FP32 sphere medium (seed 42: 200 spheres, radius
+U[4,20] cells, eps_r U[1,12], vacuum background), dx = 1, dt = 0.99/sqrt(3),
+PEC walls, Gaussian-modulated Jz point source at the centre
+J(n) = exp(-((n-30)/10)^2) sin(2 pi 0.05 n), FP64 DFT monitor of Ex and Ez on
+the z = 256 plane at 0.03/0.05/0.07 cycles/step, 1000 time steps.
+
+One benchmark "step" = one complete 1000-time-step run of that workload on
+device-resident inputs (fields reset to zero before each run, outside the
+timed region). value = 512^3 * 1000 * K / (wall time of the K timed runs).
+
+Prints one JSON line (see the keys below). ``--gpus`` above 1 is refused: this
+build measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_ALG_FP32 = 52  # bytes per cell-update: 6 fields x 4 B x (read + write) + one FP32 coefficient
+HBM_PEAK_GBS = 8000.0
+METRIC = None
+
+
+def load_metric():
+    try:
+        with open(os.path.join(ROOT, "BASELINE.json")) as f:
+            return json.load(f)["metric"]
+    except Exception:
+        return "Gcell-updates/s at 512^3 + % HBM roofline"
+
+
+def parse_traffic(kernel_key):
+    """HBM bytes per launch of the stepping kernel, from the committed rocprofv3
+    PMC summary (profiles/<kernel>_traffic.txt, line 'bytes_per_launch: X')."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_key}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("bytes_per_launch:"):
+                try:
+                    return float(ln.split(":", 1)[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def build_problem(gd, n, timesteps, seed=42):
+    dt = np.float32(0.99 / math.sqrt(3.0))
+    eps = gd.sphere_medium(n, n, n, seed, 200, 4.0, 20.0, 1.0, 12.0)
+    src = gd.gaussian_pulse_table(timesteps, 30.0, 10.0, 0.05)
+    return dt, eps, src
+
+
+def cpu_baseline(gd, n, eps, dt, src, steps, threads):
+    """Oracle (C port of update_FDTD_omp, gdiamond_omp.hpp:110-172) on the host
+    cores for a bounded number of steps of the same workload."""
+    from oracle import oracle as O
+    cb, db = gd.iso_cb_from_eps(eps, dt, 1.0)
+    shape = (n, n, n)
+    f = O.zero_fields(shape)
+    c = n // 2
+    # warm the pages once (first touch) outside the timing
+    O.run(shape, f, cb=cb, db=db, dx=1.0, sources=[(2, c, c, c, src)], nsteps=1,
+          threads=threads)
+    t0 = time.perf_counter()
+    O.run(shape, f, cb=cb, db=db, dx=1.0, sources=[(2, c, c, c, src)], n0=1, nsteps=steps,
+          threads=threads)
+    dt_s = time.perf_counter() - t0
+    return n ** 3 * steps / dt_s / 1e9
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed runs of the whole workload")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--timesteps", type=int, default=1000)
+    ap.add_argument("--variant", choices=["tiled", "plain"], default="tiled")
+    ap.add_argument("--blt", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only the copy kernel used to calibrate HBM counters")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not a measured path)"}))
+        sys.exit(2)
+
+    import gdiamond_amd as gd
+
+    n, T = args.size, args.timesteps
+    if args.calibrate_copy:
+        ctx = gd.Context(3, 3, 3)
+        nfl = 256 * 1024 * 1024
+        ms = ctx.calibration_copy(nfl, reps=5)
+        print(json.dumps({"copy_floats": nfl, "bytes_read": 4 * nfl, "bytes_written": 4 * nfl,
+                          "ms": ms, "GBps": 8 * nfl / ms / 1e6}))
+        return
+
+    t_setup = time.perf_counter()
+    dt, eps, src = build_problem(gd, n, T)
+    ctx = gd.Context(n, n, n, dx=1.0)
+    ctx.set_isotropic_eps(eps, dt)
+    c = n // 2
+    ctx.add_source(gd.EZ, c, c, c, src)
+    f_cps = np.array([0.03, 0.05, 0.07])
+    ctx.dft_config([gd.EX, gd.EZ], axis=2, index=c, f_cps=f_cps)
+    ctx.set_launch_timing(True)
+    variant = gd.TILED if args.variant == "tiled" else gd.PLAIN
+    tiling = None
+    if variant == gd.TILED:
+        tiling = ctx.plan_tiling()
+        for k in ("blt", "rows", "waves", "zchunk"):
+            v = getattr(args, k)
+            if v:
+                tiling[k] = v
+    setup_s = time.perf_counter() - t_setup
+    print(f"# setup {setup_s:.1f}s tiling={tiling}", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        ctx.reset()
+        ctx.run(T, variant, tiling)
+        print(f"# warmup {w} {ctx.last_run_stats()}", file=sys.stderr, flush=True)
+
+    wall = []
+    kernel_ms = []
+    nlaunch = 0
+    for s in range(args.steps):
+        ctx.reset()
+        t0 = time.perf_counter()
+        ctx.run(T, variant, tiling)
+        wall.append(time.perf_counter() - t0)
+        st = ctx.last_run_stats()
+        kernel_ms.append(st["kernel_ms"])
+        nlaunch = st["nlaunch"]
+        print(f"# step {s} wall {wall[-1]*1e3:.1f} ms {st}", file=sys.stderr, flush=True)
+
+    # sanity: the monitor saw the pulse, fields are finite
+    re, im = ctx.dft_get(gd.EZ)
+    finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
+
+    cells = n ** 3
+    total = sum(wall)
+    value = cells * T * args.steps / total / 1e9
+    ms_per_step = total / args.steps * 1e3
+    mean_launch_ms = sum(kernel_ms) / max(1, nlaunch * args.steps)
+    # algorithmic bytes of all launches of one run / their summed duration
+    achieved = B_ALG_FP32 * cells * T / (np.mean(kernel_ms) * 1e-3) / 1e9
+    kernel_key = "zstream" if variant == gd.TILED else "plain"
+    traffic = parse_traffic(kernel_key) if variant == gd.TILED else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "zstream_kernel" if variant == gd.TILED else "plain_e/h_kernel",
+                "bytes_alg_per_cell_update": B_ALG_FP32,
+                "mean_launch_ms": round(mean_launch_ms, 4),
+                "launches_per_run": nlaunch}
+
+    out = {
+        "metric": load_metric(),
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"fdtd3d-yee-spheres-{n}^3-{T}steps-dft3",
+                   "grid": [n, n, n], "timesteps": T, "medium": "200 spheres seed 42, eps 1-12",
+                   "source": "Gaussian Jz at centre", "dft": "Ex,Ez z=256 f=0.03/0.05/0.07 fp64",
+                   "variant": args.variant, "tiling": tiling, "parallelism": "single-gpu",
+                   "dft_finite": finite},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(gd, n, eps, dt, src, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": round(cpu, 4), "unit": "Gcell-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"same {n}^3 workload, {args.cpu_steps} of {T} steps, "
+                                         f"OpenMP collapse(2) port of update_FDTD_omp"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,328 @@
+"""gdiamond_amd -- MI355X (gfx950) 3D Yee FDTD time-stepping library.
+
+Python binding of the C ABI in include/gdiamond/gd_capi.h (library
+``gdiamond_amd/libgdiamond_hip.so``, built in-tree by ``gdiamond_amd.build``).
+There is no CPU fallback: every stepping call runs the HIP kernels, and
+importing a missing library raises.
+
+The reference's GPU path is the member functions of ``gdiamond::gDiamond``
+(gDiamond/gdiamond.hpp:76-177); its C++ drop-in is include/gdiamond/gdiamond.hpp.
+This module exposes the same device state to Python as :class:`Context`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgdiamond_hip.so")
+
+EX, EY, EZ, HX, HY, HZ = range(6)
+COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+COEFF_NAMES = ("Cax", "Cbx", "Cay", "Cby", "Caz", "Cbz",
+               "Dax", "Dbx", "Day", "Dby", "Daz", "Dbz")
+PLAIN, TILED = 0, 1
+
+GD_OK = 0
+ERRORS = {-1: "GD_ERR_ARG", -2: "GD_ERR_STATE", -3: "GD_ERR_HIP", -4: "GD_ERR_NOMEM",
+          -5: "GD_ERR_UNSUPPORTED", -6: "GD_ERR_NODEVICE"}
+
+
+class GdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Tiling(C.Structure):
+    _fields_ = [("blt", C.c_int), ("rows", C.c_int), ("waves", C.c_int), ("zchunk", C.c_int)]
+
+    def as_dict(self):
+        return {"blt": self.blt, "rows": self.rows, "waves": self.waves, "zchunk": self.zchunk}
+
+
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "gd_version": (C.c_char_p, []),
+    "gd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "gd_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.c_size_t, C.c_size_t,
+                            C.c_float]),
+    "gd_destroy": (C.c_int, [C.c_void_p]),
+    "gd_last_error": (C.c_char_p, [C.c_void_p]),
+    "gd_set_coefficients": (C.c_int, [C.c_void_p, _FP * 12]),
+    "gd_set_isotropic": (C.c_int, [C.c_void_p, _FP, C.c_float]),
+    "gd_set_isotropic_eps": (C.c_int, [C.c_void_p, _FP, C.c_float]),
+    "gd_add_source": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, _FP,
+                                C.c_size_t]),
+    "gd_clear_sources": (C.c_int, [C.c_void_p]),
+    "gd_set_fields": (C.c_int, [C.c_void_p, _FP * 6]),
+    "gd_get_fields": (C.c_int, [C.c_void_p, _FP * 6]),
+    "gd_get_field": (C.c_int, [C.c_void_p, C.c_int, _FP]),
+    "gd_get_plane_xy": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, _FP]),
+    "gd_reset": (C.c_int, [C.c_void_p]),
+    "gd_get_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "gd_dft_config": (C.c_int, [C.c_void_p, C.c_uint, C.c_int, C.c_size_t, _DP, C.c_size_t,
+                                C.c_double]),
+    "gd_dft_get": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
+    "gd_dft_disable": (C.c_int, [C.c_void_p]),
+    "gd_run": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.POINTER(Tiling)]),
+    "gd_plan_tiling": (C.c_int, [C.c_void_p, C.POINTER(Tiling)]),
+    "gd_last_run_stats": (C.c_int, [C.c_void_p, _DP, _DP, C.POINTER(C.c_size_t)]),
+    "gd_set_launch_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "gd_calibration_copy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _DP]),
+    "gd_default_problem_coefficients": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, _FP * 12]),
+    "gd_default_mz_table": (C.c_int, [C.c_size_t, _FP]),
+    "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
+                                   C.c_double, C.c_double, C.c_double, C.c_double, _FP]),
+    "gd_gaussian_pulse_table": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, _FP]),
+}
+
+
+def exported_symbols():
+    """Names of every C-ABI entry point this binding expects in the library."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Load libgdiamond_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m gdiamond_amd.build` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def version() -> str:
+    return lib().gd_version().decode()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().gd_device_count(C.byref(n))
+    return n.value
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} elements, got {a.size}")
+    return a
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _check_host(rc, what="host call"):
+    if rc != GD_OK:
+        raise GdError(rc, what)
+
+
+# ---------------------------------------------------------------- host builders
+
+
+def default_problem_coefficients(nx, ny, nz):
+    """Reference default problem (gdiamond.hpp:20-73, utils.h:542-690): 12 arrays (nz,ny,nx)."""
+    out = [np.empty((nz, ny, nx), np.float32) for _ in range(12)]
+    _check_host(lib().gd_default_problem_coefficients(nx, ny, nz,
+                                                      (_FP * 12)(*[_fp(a) for a in out])))
+    return out
+
+
+def default_mz_table(nsteps):
+    """Reference default source Mz(t) (gdiamond.hpp:1189-1191), t = 0..nsteps-1."""
+    out = np.empty(nsteps, np.float32)
+    _check_host(lib().gd_default_mz_table(nsteps, _fp(out)))
+    return out
+
+
+def sphere_medium(nx, ny, nz, seed, nspheres=200, rmin=4.0, rmax=20.0, emin=1.0, emax=12.0):
+    """Seeded random dielectric spheres in vacuum; eps_r array shaped (nz, ny, nx)."""
+    out = np.empty((nz, ny, nx), np.float32)
+    _check_host(lib().gd_sphere_medium(nx, ny, nz, seed, nspheres, rmin, rmax, emin, emax,
+                                       _fp(out)))
+    return out
+
+
+def gaussian_pulse_table(nsteps, t0=30.0, tau=10.0, f_cps=0.05):
+    """J(n) = exp(-((n-t0)/tau)^2) sin(2 pi f n), double then float."""
+    out = np.empty(nsteps, np.float32)
+    _check_host(lib().gd_gaussian_pulse_table(nsteps, t0, tau, f_cps, _fp(out)))
+    return out
+
+
+def iso_cb_from_eps(eps_r, dt, dx):
+    """The library's isotropic coefficients: cb = dt/(eps*dx), db = dt/dx (double, then float)."""
+    eps = np.asarray(eps_r, np.float64)
+    cb = (np.float64(dt) / (eps * np.float64(dx))).astype(np.float32)
+    db = np.float32(np.float64(dt) / np.float64(dx))
+    return cb, db
+
+
+# ---------------------------------------------------------------- device context
+
+
+class Context:
+    """One device-resident FDTD problem (a ``gd_ctx``)."""
+
+    def __init__(self, nx, ny, nz, dx=1.0):
+        self.shape = (int(nx), int(ny), int(nz))
+        self.N = int(nx) * int(ny) * int(nz)
+        h = C.c_void_p()
+        rc = lib().gd_create(C.byref(h), nx, ny, nz, dx)
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_last_error(None).decode())
+        self._h = h
+        self._dft = None
+
+    # -- plumbing
+    def _ok(self, rc):
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def zyx(self):
+        nx, ny, nz = self.shape
+        return (nz, ny, nx)
+
+    # -- material
+    def set_coefficients(self, coeffs):
+        arrs = [_f32(c, self.N) for c in coeffs]
+        if len(arrs) != 12:
+            raise ValueError("need 12 coefficient arrays")
+        self._ok(lib().gd_set_coefficients(self._h, (_FP * 12)(*[_fp(a) for a in arrs])))
+
+    def set_isotropic(self, cb, db):
+        a = _f32(cb, self.N)
+        self._ok(lib().gd_set_isotropic(self._h, _fp(a), float(db)))
+
+    def set_isotropic_eps(self, eps_r, dt):
+        a = _f32(eps_r, self.N)
+        self._ok(lib().gd_set_isotropic_eps(self._h, _fp(a), float(dt)))
+
+    # -- sources
+    def add_source(self, comp, i, j, k, table):
+        t = _f32(table)
+        self._ok(lib().gd_add_source(self._h, int(comp), i, j, k, _fp(t), t.size))
+
+    def clear_sources(self):
+        self._ok(lib().gd_clear_sources(self._h))
+
+    # -- fields
+    def set_fields(self, fields):
+        arrs = [_f32(f, self.N) for f in fields]
+        self._ok(lib().gd_set_fields(self._h, (_FP * 6)(*[_fp(a) for a in arrs])))
+
+    def get_fields(self):
+        out = [np.empty(self.zyx, np.float32) for _ in range(6)]
+        self._ok(lib().gd_get_fields(self._h, (_FP * 6)(*[_fp(a) for a in out])))
+        return out
+
+    def get_field(self, comp):
+        out = np.empty(self.zyx, np.float32)
+        self._ok(lib().gd_get_field(self._h, int(comp), _fp(out)))
+        return out
+
+    def get_plane_xy(self, comp, k):
+        nx, ny, _ = self.shape
+        out = np.empty((ny, nx), np.float32)
+        self._ok(lib().gd_get_plane_xy(self._h, int(comp), k, _fp(out)))
+        return out
+
+    def reset(self):
+        self._ok(lib().gd_reset(self._h))
+
+    @property
+    def step(self):
+        s = C.c_size_t()
+        self._ok(lib().gd_get_step(self._h, C.byref(s)))
+        return s.value
+
+    # -- DFT
+    def dft_config(self, comps, axis, index, f_cps, scale=1.0):
+        mask = 0
+        for c in comps:
+            mask |= 1 << int(c)
+        f = np.ascontiguousarray(f_cps, np.float64)
+        self._ok(lib().gd_dft_config(self._h, mask, int(axis), int(index), _dp(f), f.size,
+                                     float(scale)))
+        nx, ny, nz = self.shape
+        plane = {0: ny * nz, 1: nx * nz, 2: nx * ny}[int(axis)]
+        self._dft = (sorted(int(c) for c in comps), f.size, plane)
+
+    def dft_get(self, comp):
+        _, nf, plane = self._dft
+        re = np.empty((nf, plane), np.float64)
+        im = np.empty((nf, plane), np.float64)
+        self._ok(lib().gd_dft_get(self._h, int(comp), _dp(re), _dp(im)))
+        return re, im
+
+    def dft_disable(self):
+        self._ok(lib().gd_dft_disable(self._h))
+        self._dft = None
+
+    # -- stepping
+    def run(self, nsteps, variant=TILED, tiling=None):
+        t = None
+        if tiling is not None:
+            t = Tiling(**tiling) if isinstance(tiling, dict) else tiling
+        self._ok(lib().gd_run(self._h, int(variant), int(nsteps),
+                              C.byref(t) if t is not None else None))
+
+    def plan_tiling(self):
+        t = Tiling()
+        self._ok(lib().gd_plan_tiling(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_launch_timing(self, enable=True):
+        self._ok(lib().gd_set_launch_timing(self._h, 1 if enable else 0))
+
+    def last_run_stats(self):
+        tot, ker, nl = C.c_double(), C.c_double(), C.c_size_t()
+        self._ok(lib().gd_last_run_stats(self._h, C.byref(tot), C.byref(ker), C.byref(nl)))
+        return {"total_ms": tot.value, "kernel_ms": ker.value, "nlaunch": nl.value}
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        self._ok(lib().gd_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def calibration_copy(self, n, reps=10):
+        ms = C.c_double()
+        self._ok(lib().gd_calibration_copy(self._h, n, reps, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,414 @@
+// TILED stepping kernel for gfx950: 2.5-D temporal blocking.
+//
+// Replaces the reference's mix-mapping tilings (ver2 diamond,
+// kernels_mm_ver2.cuh:488-720; ver3 overlapped "replication" mountains,
+// kernels_mm_ver3.cuh:417-647). Those load one 16x8x8 block of all six fields
+// into shared memory and run BLT sub-steps on it. Here a workgroup owns an
+// overlapped x-y tile and streams it along z through BLT time levels kept in
+// registers, so each field value is read from HBM once per launch and written
+// once, whatever BLT is, and redundant compute exists only in x and y.
+//
+// Geometry of one workgroup (template B = time levels, R = rows per thread,
+// W = waves):
+//   * input tile 64 x (W*R) columns; lane = x (one wave64 per x row segment),
+//     each thread holds R consecutive y rows of its lane in registers;
+//   * owned output 64-2B x W*R-2B columns (the halo shrinks by one cell per
+//     side per level: E needs H at i-1/j-1, H needs E at i+1/j+1);
+//   * z: owned planes [z_lo, z_hi); streaming starts B planes below z_lo.
+// Level b (1..B) runs one plane behind level b-1: at stream position p it
+// updates E of step n0+b-1 on plane p-b+1 and then H on plane p-b.
+// Neighbours: x via DPP lane shifts across the wave (wave_shr/wave_shl), y
+// within a thread's rows from registers and across waves through a small LDS
+// exchange of one row, z from the previous plane kept in registers.
+// Physical boundary cells (index 0 or n-1 on any axis) are frozen, exactly as
+// the reference updates only 1..n-2 (gdiamond.hpp:1194-1221).
+//
+// Validity: a cell of level b is exact when its whole dependency cone lies in
+// the loaded tile; owned cells of level B always do. Cells outside the cone
+// (tile rim, planes below the chunk, columns outside the grid) hold garbage
+// that only ever feeds other cells outside the cone, so loads are clamped into
+// the grid instead of predicated.
+#include "gd_common.hpp"
+
+namespace gd {
+
+namespace {
+
+struct ZArgs {
+  Fields src;
+  Fields dst;
+  Material m;
+  Grid g;
+  Sources s;
+  Dft d;
+  int n0;         // absolute step of level 1
+  int run_step0;  // DFT weight column of level 1
+  int ntx, nty, ntz;
+  int zc;         // planes owned per workgroup
+};
+
+// Lane l receives the value of lane l-1 (lane 0 receives 0): wave_shr:1.
+__device__ __forceinline__ float from_lane_below(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+// Lane l receives the value of lane l+1 (lane 63 receives 0): wave_shl:1.
+__device__ __forceinline__ float from_lane_above(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs. Give each XCD a
+// contiguous run of tiles (x fastest, then y, then z) so tiles that share
+// halos share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Subtract the point-source terms of step n on plane z from the curl
+// brackets c0..c2 of components base..base+2 (only for columns flagged in
+// `mask`; at most one source per cell and component).
+__device__ __forceinline__ void apply_sources(const Sources& s, unsigned mask, int base, int z,
+                                              int n, float dx, float& c0, float& c1, float& c2) {
+  for (int q = 0; q < s.n; ++q) {
+    if (!((mask >> q) & 1u) || s.k[q] != z) continue;
+    const int c = s.comp[q] - base;
+    if (c < 0 || c > 2) continue;
+    const float v = s.table[q][n];
+    if (c == 0) c0 = c0 - v * dx;
+    if (c == 1) c1 = c1 - v * dx;
+    if (c == 2) c2 = c2 - v * dx;
+  }
+}
+
+template <int B, int R, int W, bool kFull>
+__global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
+  constexpr int OX = 64 - 2 * B;
+  constexpr int OY = W * R - 2 * B;
+  // lds_h[level-1][parity of p][wave][Hz,Hx][lane]: last row of H of a level,
+  // read by the next wave for the j-1 neighbour of the next level.
+  // lds_e[level-1][parity of plane][wave][Ez,Ex][lane]: first row of E of a
+  // level, read by the previous wave one stream position later (j+1).
+  __shared__ float lds_h[B][2][W][2][64];
+  __shared__ float lds_e[B][2][W][2][64];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % a.ntx;
+  const int tyz = tile / a.ntx;
+  const int ty = tyz % a.nty;
+  const int tz = tyz / a.nty;
+
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  const size_t sxy = (size_t)a.g.sxy;
+  const float dx = a.g.dx;
+
+  const int gx = tx * OX - B + lane;
+  const int gy0 = ty * OY - B + w * R;
+  const bool intx = gx >= 1 && gx <= nx - 2;
+  const bool ownx = lane >= B && lane < B + OX && gx < nx;
+  const unsigned cgx = (unsigned)clampi(gx, 0, nx - 1);
+  bool upd_xy[R], own[R];
+  unsigned col[R];     // in-plane offset, clamped into the grid
+  unsigned srow[R];    // point sources in this column (bit q = source q)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gy = gy0 + r;
+    upd_xy[r] = intx && gy >= 1 && gy <= ny - 2;
+    const int ly = w * R + r;
+    own[r] = ownx && ly >= B && ly < B + OY && gy < ny;
+    col[r] = cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx;
+    srow[r] = 0u;
+    for (int q = 0; q < a.s.n; ++q)
+      if (a.s.i[q] == gx && a.s.j[q] == gy) srow[r] |= 1u << q;
+  }
+  const unsigned colm = cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx;
+
+  const int z_lo = tz * a.zc;
+  const int z_hi = min(nz, z_lo + a.zc);
+  const int p_begin = max(0, z_lo - B);
+  const int p_end = z_hi - 1 + B;
+
+  // Pipeline state (registers):
+  //   eq[b] = E of level b on plane p-b   (b = 1..B, slot 0 unused)
+  //   hq[b] = H of level b on plane p-b-1 (b = 0..B-1; level 0 = loaded H)
+  //   cbq[b-1] = Cb on plane p-b+1, used by level b (isotropic material)
+  float eq[B + 1][R][3];
+  float hq[B][R][3];
+  float cbq[B][R];
+#pragma unroll
+  for (int b = 0; b <= B; ++b)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        eq[b][r][c] = 0.0f;
+        if (b < B) hq[b][r][c] = 0.0f;
+      }
+#pragma unroll
+  for (int b = 0; b < B; ++b)
+#pragma unroll
+    for (int r = 0; r < R; ++r) cbq[b][r] = 0.0f;
+
+  const bool dft_on = a.d.ncomp > 0;
+
+  // Exchange rows start at zero; they are only consumed by cells outside the
+  // valid cone before the first real write, but stay deterministic.
+  for (int q = threadIdx.x; q < B * 2 * W * 2 * 64; q += 64 * W) {
+    (&lds_h[0][0][0][0][0])[q] = 0.0f;
+    (&lds_e[0][0][0][0][0])[q] = 0.0f;
+  }
+  __syncthreads();
+
+  for (int p = p_begin; p <= p_end; ++p) {
+    // ---- level 0: plane p of the source copy (clamped; planes >= nz only
+    // feed cells outside the cone)
+    float hc[R][3], ec[R][3];
+    {
+      const size_t pz = (size_t)min(p, nz - 1) * sxy;
+      const float* __restrict__ s0 = a.src.c[0] + pz;
+      const float* __restrict__ s1 = a.src.c[1] + pz;
+      const float* __restrict__ s2 = a.src.c[2] + pz;
+      const float* __restrict__ s3 = a.src.c[3] + pz;
+      const float* __restrict__ s4 = a.src.c[4] + pz;
+      const float* __restrict__ s5 = a.src.c[5] + pz;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        ec[r][0] = s0[col[r]];
+        ec[r][1] = s1[col[r]];
+        ec[r][2] = s2[col[r]];
+        hc[r][0] = s3[col[r]];
+        hc[r][1] = s4[col[r]];
+        hc[r][2] = s5[col[r]];
+      }
+      if (!kFull) {
+        const float* __restrict__ cbp = a.m.cb + pz;
+#pragma unroll
+        for (int b = B - 1; b > 0; --b)
+#pragma unroll
+          for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
+#pragma unroll
+        for (int r = 0; r < R; ++r) cbq[0][r] = cbp[col[r]];
+      }
+    }
+    float hmz, hmx;  // H of the previous level on row gy0-1 (j-1 of row 0)
+    {
+      const size_t pz = (size_t)min(p, nz - 1) * sxy;
+      hmz = a.src.c[5][pz + colm];
+      hmx = a.src.c[3][pz + colm];
+    }
+
+#pragma unroll
+    for (int b = 1; b <= B; ++b) {
+      const int zE = p - b + 1;
+      const int zH = p - b;
+      const int n = a.n0 + b - 1;
+      const bool intzE = zE >= 1 && zE <= nz - 2;
+      const bool intzH = zH >= 1 && zH <= nz - 2;
+
+      // ---- E of step n on plane zE from H(level b-1) on zE (hc) and zE-1 (hq[b-1])
+      float en[R][3];
+      {
+        const size_t oE = (size_t)clampi(zE, 0, nz - 1) * sxy;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float hz_xm = from_lane_below(hc[r][2]);
+          const float hy_xm = from_lane_below(hc[r][1]);
+          const float hz_ym = r > 0 ? hc[r - 1][2] : hmz;
+          const float hx_ym = r > 0 ? hc[r - 1][0] : hmx;
+          const bool upd = upd_xy[r] && intzE;
+          float cx = (hc[r][2] - hz_ym) - (hc[r][1] - hq[b - 1][r][1]);
+          float cy = (hc[r][0] - hq[b - 1][r][0]) - (hc[r][2] - hz_xm);
+          float cz = (hc[r][1] - hy_xm) - (hc[r][0] - hx_ym);
+          if (srow[r]) apply_sources(a.s, srow[r], 0, zE, n, dx, cx, cy, cz);
+          if (kFull) {
+            const size_t q = oE + col[r];
+            en[r][0] = upd ? a.m.c[0][q] * ec[r][0] + a.m.c[1][q] * cx : ec[r][0];
+            en[r][1] = upd ? a.m.c[2][q] * ec[r][1] + a.m.c[3][q] * cy : ec[r][1];
+            en[r][2] = upd ? a.m.c[4][q] * ec[r][2] + a.m.c[5][q] * cz : ec[r][2];
+          } else {
+            const float cb = cbq[b - 1][r];
+            en[r][0] = upd ? ec[r][0] + cb * cx : ec[r][0];
+            en[r][1] = upd ? ec[r][1] + cb * cy : ec[r][1];
+            en[r][2] = upd ? ec[r][2] + cb * cz : ec[r][2];
+          }
+        }
+      }
+
+      // ---- H of step n on plane zH from E(level b) on zH (eq[b]) and zH+1 (en)
+      float hn[R][3];
+      {
+        float ez_last = 0.0f, ex_last = 0.0f;
+        if (w + 1 < W) {
+          ez_last = lds_e[b - 1][zH & 1][w + 1][0][lane];
+          ex_last = lds_e[b - 1][zH & 1][w + 1][1][lane];
+        }
+        const size_t oH = (size_t)clampi(zH, 0, nz - 1) * sxy;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float ez_xp = from_lane_above(eq[b][r][2]);
+          const float ey_xp = from_lane_above(eq[b][r][1]);
+          const float ez_yp = r + 1 < R ? eq[b][r + 1][2] : ez_last;
+          const float ex_yp = r + 1 < R ? eq[b][r + 1][0] : ex_last;
+          const bool upd = upd_xy[r] && intzH;
+          float cx = (en[r][1] - eq[b][r][1]) - (ez_yp - eq[b][r][2]);
+          float cy = (ez_xp - eq[b][r][2]) - (en[r][0] - eq[b][r][0]);
+          float cz = (ex_yp - eq[b][r][0]) - (ey_xp - eq[b][r][1]);
+          if (srow[r]) apply_sources(a.s, srow[r], 3, zH, n, dx, cx, cy, cz);
+          if (kFull) {
+            const size_t q = oH + col[r];
+            hn[r][0] = upd ? a.m.c[6][q] * hq[b - 1][r][0] + a.m.c[7][q] * cx : hq[b - 1][r][0];
+            hn[r][1] = upd ? a.m.c[8][q] * hq[b - 1][r][1] + a.m.c[9][q] * cy : hq[b - 1][r][1];
+            hn[r][2] = upd ? a.m.c[10][q] * hq[b - 1][r][2] + a.m.c[11][q] * cz : hq[b - 1][r][2];
+          } else {
+            const float db = a.m.db;
+            hn[r][0] = upd ? hq[b - 1][r][0] + db * cx : hq[b - 1][r][0];
+            hn[r][1] = upd ? hq[b - 1][r][1] + db * cy : hq[b - 1][r][1];
+            hn[r][2] = upd ? hq[b - 1][r][2] + db * cz : hq[b - 1][r][2];
+          }
+        }
+      }
+
+      // ---- DFT of the values after step n (owned cells of owned planes)
+      if (dft_on) {
+        const int rs = a.run_step0 + b - 1;
+        const bool okE = zE >= z_lo && zE < z_hi;
+        const bool okH = zH >= z_lo && zH < z_hi;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (!own[r]) continue;
+          const long long pe = okE ? dft_plane_index(a.d, a.g, gx, gy0 + r, zE) : -1;
+          const long long ph = okH ? dft_plane_index(a.d, a.g, gx, gy0 + r, zH) : -1;
+          if (pe < 0 && ph < 0) continue;
+          for (int q = 0; q < a.d.ncomp; ++q) {
+            const int c = a.d.comp[q];
+            if (c < 3 && pe >= 0)
+              dft_accumulate(a.d, q, pe, c == 0 ? en[r][0] : (c == 1 ? en[r][1] : en[r][2]), rs);
+            if (c >= 3 && ph >= 0)
+              dft_accumulate(a.d, q, ph, c == 3 ? hn[r][0] : (c == 4 ? hn[r][1] : hn[r][2]), rs);
+          }
+        }
+      }
+
+      // ---- publish boundary rows for the neighbouring waves
+      lds_e[b - 1][zE & 1][w][0][lane] = en[0][2];
+      lds_e[b - 1][zE & 1][w][1][lane] = en[0][0];
+      if (b < B) {
+        lds_h[b - 1][p & 1][w][0][lane] = hn[R - 1][2];
+        lds_h[b - 1][p & 1][w][1][lane] = hn[R - 1][0];
+      }
+
+      // ---- final level: owned cells go to the destination copy
+      if (b == B) {
+        if (zE >= z_lo && zE < z_hi) {
+          const size_t oE = (size_t)zE * sxy;
+          float* __restrict__ d0 = a.dst.c[0] + oE;
+          float* __restrict__ d1 = a.dst.c[1] + oE;
+          float* __restrict__ d2 = a.dst.c[2] + oE;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (own[r]) {
+              d0[col[r]] = en[r][0];
+              d1[col[r]] = en[r][1];
+              d2[col[r]] = en[r][2];
+            }
+        }
+        if (zH >= z_lo && zH < z_hi) {
+          const size_t oH = (size_t)zH * sxy;
+          float* __restrict__ d3 = a.dst.c[3] + oH;
+          float* __restrict__ d4 = a.dst.c[4] + oH;
+          float* __restrict__ d5 = a.dst.c[5] + oH;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (own[r]) {
+              d3[col[r]] = hn[r][0];
+              d4[col[r]] = hn[r][1];
+              d5[col[r]] = hn[r][2];
+            }
+        }
+      }
+
+      // ---- advance: level b+1 consumes H(b) on zE-1 (hn), H(b) on zE-2
+      // (hq[b], still the previous plane) and E(b) on zE-1 (eq[b] before the
+      // shift).
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float e_prev = eq[b][r][c];
+          eq[b][r][c] = en[r][c];
+          hq[b - 1][r][c] = hc[r][c];
+          hc[r][c] = hn[r][c];
+          ec[r][c] = e_prev;
+        }
+      if (b < B || B == 1) __syncthreads();
+      if (b < B) {
+        hmz = w > 0 ? lds_h[b - 1][p & 1][w - 1][0][lane] : 0.0f;
+        hmx = w > 0 ? lds_h[b - 1][p & 1][w - 1][1][lane] : 0.0f;
+      }
+    }
+  }
+}
+
+template <int B, int R, int W>
+hipError_t launch_one(const ZArgs& a, bool full, hipStream_t stream) {
+  const unsigned blocks = (unsigned)((long long)a.ntx * a.nty * a.ntz);
+  if (full)
+    hipLaunchKernelGGL((zstream_kernel<B, R, W, true>), dim3(blocks), dim3(64 * W), 0, stream, a);
+  else
+    hipLaunchKernelGGL((zstream_kernel<B, R, W, false>), dim3(blocks), dim3(64 * W), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int R, int W>
+hipError_t launch_rw(int blt, const ZArgs& a, bool full, hipStream_t stream) {
+  switch (blt) {
+    case 1: return launch_one<1, R, W>(a, full, stream);
+    case 2: return launch_one<2, R, W>(a, full, stream);
+    case 3: return launch_one<3, R, W>(a, full, stream);
+    case 4: return launch_one<4, R, W>(a, full, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool ztiling_supported(const ZTiling& t, bool /*full*/) {
+  if (t.blt < 1 || t.blt > 4 || t.zchunk < 1) return false;
+  const bool rw = (t.rows == 2 && t.waves == 4) || (t.rows == 2 && t.waves == 8) ||
+                  (t.rows == 4 && t.waves == 4) || (t.rows == 4 && t.waves == 8);
+  if (!rw) return false;
+  return t.rows * t.waves - 2 * t.blt >= 1 && 64 - 2 * t.blt >= 1;
+}
+
+hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst,
+                          const Material& m, bool full, const Grid& g, const Sources& s,
+                          const Dft& d, int n0, int run_step0, hipStream_t stream) {
+  if (!ztiling_supported(t, full)) return hipErrorInvalidValue;
+  ZArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.m = m;
+  a.g = g;
+  a.s = s;
+  a.d = d;
+  a.n0 = n0;
+  a.run_step0 = run_step0;
+  const int ox = 64 - 2 * t.blt;
+  const int oy = t.rows * t.waves - 2 * t.blt;
+  a.ntx = (g.nx + ox - 1) / ox;
+  a.nty = (g.ny + oy - 1) / oy;
+  a.zc = t.zchunk;
+  a.ntz = (g.nz + t.zchunk - 1) / t.zchunk;
+  if (t.rows == 2 && t.waves == 4) return launch_rw<2, 4>(t.blt, a, full, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_rw<2, 8>(t.blt, a, full, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_rw<4, 4>(t.blt, a, full, stream);
+  return launch_rw<4, 8>(t.blt, a, full, stream);
+}
+
+}  // namespace gd

@@ -1,0 +1,195 @@
+/*
+ * gd_capi.h -- C ABI of libgdiamond_hip.so, the MI355X (gfx950) 3D Yee FDTD
+ * time-stepping library.
+ *
+ * The reference exposes its GPU path only as members of the header-only class
+ * gdiamond::gDiamond (gDiamond/gdiamond.hpp:17-177); every member allocates,
+ * runs and frees its own device buffers. This ABI replaces those members with
+ * a context whose device state persists across calls. The C++ class with the
+ * reference's names is include/gdiamond/gdiamond.hpp, built on this ABI.
+ *
+ * Conventions
+ *  - Host arrays are structure-of-arrays, one float array per component,
+ *    linear index idx = i + j*nx + k*nx*ny (x fastest), as the reference
+ *    (gdiamond.hpp:1197). The caller owns host buffers; the library copies.
+ *  - Component ids: 0..5 = Ex Ey Ez Hx Hy Hz.
+ *  - Coefficient order: Cax Cbx Cay Cby Caz Cbz Dax Dbx Day Dby Daz Dbz
+ *    (gdiamond.hpp:870-890). Cb/Db already contain 1/dx (utils.h:656-686).
+ *  - Every call returns int status: GD_OK (0) or a negative GD_ERR_*;
+ *    gd_last_error(ctx) gives the text. No call exits the process (the
+ *    reference's CUDACHECK calls exit(1), gdiamond_gpu.cuh:11-20).
+ *  - One context per host thread; calls on one context are serialised. Each
+ *    context owns one HIP stream of default priority on the device that was
+ *    current at gd_create.
+ *  - Only interior cells 1..n-2 on every axis are updated; boundary cells keep
+ *    their values (the reference's PEC walls, gdiamond.hpp:1194-1221).
+ */
+#ifndef GDIAMOND_GD_CAPI_H
+#define GDIAMOND_GD_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gd_ctx gd_ctx;
+
+enum {
+  GD_OK = 0,
+  GD_ERR_ARG = -1,         /* bad argument (size, index, null pointer, range) */
+  GD_ERR_STATE = -2,       /* call not valid in the current state (e.g. no material) */
+  GD_ERR_HIP = -3,         /* HIP runtime error (text in gd_last_error) */
+  GD_ERR_NOMEM = -4,       /* device or host allocation failed */
+  GD_ERR_UNSUPPORTED = -5, /* unsupported configuration */
+  GD_ERR_NODEVICE = -6     /* no HIP device visible */
+};
+
+enum { GD_EX = 0, GD_EY = 1, GD_EZ = 2, GD_HX = 3, GD_HY = 4, GD_HZ = 5 };
+
+/* Stepping kernels.
+ * GD_VARIANT_PLAIN: two launches per step (E kernel, then H kernel), fields
+ *   updated in place. Replaces updateE/H_3Dmap_fix (kernels.cuh:337-394) and
+ *   updateE/H_2Dmap (kernels.cuh:207-264) behind
+ *   update_FDTD_gpu_3D_warp_underutilization_fix / update_FDTD_gpu_2D
+ *   (gdiamond_gpu.cuh:809, :483).
+ * GD_VARIANT_TILED: temporally tiled; one launch advances `blt` steps over
+ *   overlapped xy tiles streamed along z, ping-ponging between two field
+ *   copies. Replaces the mix-mapping kernels ver2/ver3
+ *   (kernels_mm_ver2.cuh:488-720, kernels_mm_ver3.cuh:417-647) behind
+ *   update_FDTD_mix_mapping_gpu_ver2/ver3 (gdiamond_gpu_mm_ver2.cuh:837,
+ *   gdiamond_gpu_mm_ver3.cuh:478). Results are bit-identical to PLAIN. */
+enum { GD_VARIANT_PLAIN = 0, GD_VARIANT_TILED = 1 };
+
+/* Tiling of GD_VARIANT_TILED. Zero fields select the planner's choice.
+ *  blt    : time steps per launch (1..GD_MAX_BLT)
+ *  rows   : y rows held by each thread
+ *  waves  : wave64 wavefronts per workgroup (tile height = rows*waves)
+ *  zchunk : z planes owned by one workgroup */
+typedef struct gd_tiling {
+  int blt;
+  int rows;
+  int waves;
+  int zchunk;
+} gd_tiling;
+
+#define GD_MAX_BLT 8
+#define GD_MAX_SOURCES 8
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Create a context for an nx*ny*nz grid (each >= 3) with cell size dx (used
+ * only in the source term "- J*dx", gdiamond.hpp:1200-1219). Fields start at 0.
+ * Replaces gDiamond::gDiamond(Nx,Ny,Nz) (gdiamond.hpp:20) for device state. */
+int gd_create(gd_ctx** out, size_t nx, size_t ny, size_t nz, float dx);
+int gd_destroy(gd_ctx* ctx);
+const char* gd_last_error(const gd_ctx* ctx);
+/* Library version string, e.g. "gdiamond-hip 0.1 gfx950". */
+const char* gd_version(void);
+
+/* ---- material ------------------------------------------------------------ */
+
+/* Full reference coefficient arrays (12 arrays of nx*ny*nz floats), the layout
+ * every reference driver copies to the device (gdiamond_gpu.cuh:871-887). */
+int gd_set_coefficients(gd_ctx* ctx, const float* const coeff[12]);
+/* Isotropic lossless form: Ca = Da = 1, one Cb array shared by Ex/Ey/Ez, one
+ * scalar Db. Bit-identical to gd_set_coefficients with those values. */
+int gd_set_isotropic(gd_ctx* ctx, const float* cb, float db);
+/* Isotropic form from relative permittivity: cb = dt/(eps_r*dx),
+ * db = dt/dx, each evaluated in double and rounded to float once. */
+int gd_set_isotropic_eps(gd_ctx* ctx, const float* eps_r, float dt);
+
+/* ---- sources --------------------------------------------------------------- */
+
+/* Point source on `component` at cell (i,j,k): at step n the update of that
+ * component subtracts table[n]*dx inside the curl bracket, exactly where the
+ * reference subtracts J[idx]*dx / M[idx]*dx (gdiamond.hpp:1200-1219). The
+ * table covers absolute steps 0..nsteps-1; a run past its end is an error.
+ * At most GD_MAX_SOURCES sources. Replaces the per-step source copies of
+ * gdiamond_gpu.cuh:568-570 and :167-170. */
+int gd_add_source(gd_ctx* ctx, int component, size_t i, size_t j, size_t k,
+                  const float* table, size_t nsteps);
+int gd_clear_sources(gd_ctx* ctx);
+
+/* ---- fields ---------------------------------------------------------------- */
+
+int gd_set_fields(gd_ctx* ctx, const float* const fields[6]);
+int gd_get_fields(gd_ctx* ctx, float* const fields[6]);
+/* Copy one field component, or one xy plane (k) of it, to the host. */
+int gd_get_field(gd_ctx* ctx, int component, float* out);
+int gd_get_plane_xy(gd_ctx* ctx, int component, size_t k, float* out);
+/* Zero all fields and DFT sums; the step counter returns to 0. */
+int gd_reset(gd_ctx* ctx);
+/* Absolute index of the next step to run. */
+int gd_get_step(const gd_ctx* ctx, size_t* step);
+
+/* ---- DFT monitor ----------------------------------------------------------- */
+
+/* Running DFT of the components in comp_mask (bit c = component c) on one
+ * plane: axis 2 -> xy plane k=index (update_field_FFT_xy, kernels.cuh:87),
+ * axis 0 -> yz plane i=index (update_field_FFT_yz, kernels.cuh:60), axis 1 ->
+ * xz plane j=index. After each step n, with v the component value after step
+ * n, for every frequency f (cycles per step):
+ *     re += v * (cos(2*pi*f*n) * scale);  im -= v * (sin(2*pi*f*n) * scale)
+ * in FP64 (the reference uses phase omega*t with t = n*dt, so
+ * f = freq*dt, and FP32 sums). Plane index: axis 2 -> i + j*nx, axis 0 ->
+ * j + k*ny, axis 1 -> i + k*nx. Reconfiguring clears the sums. */
+int gd_dft_config(gd_ctx* ctx, unsigned comp_mask, int axis, size_t index,
+                  const double* f_cycles_per_step, size_t nf, double scale);
+/* re/im: nf*plane doubles each for `component` (must be in the mask). */
+int gd_dft_get(gd_ctx* ctx, int component, double* re, double* im);
+int gd_dft_disable(gd_ctx* ctx);
+
+/* ---- stepping ---------------------------------------------------------------- */
+
+/* Advance nsteps steps from the current step with the given kernel variant
+ * (tiling may be NULL). Synchronous: returns after the device finished. */
+int gd_run(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
+/* The tiling the planner would use for GD_VARIANT_TILED on this context. */
+int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
+
+/* Timing of the last gd_run, from HIP events on the context's stream:
+ * total_ms spans the whole stepping loop; with per-launch timing enabled
+ * (gd_set_launch_timing) kernel_ms is the sum of the durations of the
+ * stepping-kernel launches and nlaunch their number. */
+int gd_last_run_stats(const gd_ctx* ctx, double* total_ms, double* kernel_ms, size_t* nlaunch);
+int gd_set_launch_timing(gd_ctx* ctx, int enable);
+
+/* Bytes of device memory held by the context. */
+int gd_device_bytes(const gd_ctx* ctx, size_t* bytes);
+
+/* Plain device-to-device copy kernel over n floats of two scratch buffers the
+ * context allocates; used to calibrate HBM traffic counters (bytes read and
+ * written per launch are exactly 4*n each). Returns kernel time in ms. */
+int gd_calibration_copy(gd_ctx* ctx, size_t n, int reps, double* ms);
+
+/* Number of visible HIP devices (0 when none). */
+int gd_device_count(int* count);
+
+/* ---- host-side problem builders (no device needed) -------------------------- */
+
+/* The reference default problem: gDiamond::gDiamond coefficients
+ * (gdiamond.hpp:20-73 + set_FDTD_matrices_3D_structure, utils.h:542-690;
+ * all-false mask, graded absorber, dx = 0.1, dt = 0.05). */
+int gd_default_problem_coefficients(size_t nx, size_t ny, size_t nz, float* const coeff[12]);
+/* Reference default source values: Mz(t) = M_amp*sin(omega*t*dt) in float
+ * (gdiamond.hpp:1189-1191), t = 0..nsteps-1. */
+int gd_default_mz_table(size_t nsteps, float* out);
+
+/* Seeded random sphere medium: eps_r = 1 background; nspheres spheres with
+ * centres uniform in [0,n) per axis, radius U[rmin,rmax], eps U[emin,emax];
+ * later spheres overwrite earlier ones; a cell is inside when
+ * |(i,j,k)-c|^2 <= r^2. RNG: SplitMix64, u = (x>>11) * 2^-53, draws in the
+ * order cx, cy, cz, r, eps per sphere. */
+int gd_sphere_medium(size_t nx, size_t ny, size_t nz, uint64_t seed, int nspheres,
+                     double rmin, double rmax, double emin, double emax, float* eps_out);
+
+/* Gaussian-modulated sine J(n) = exp(-((n-t0)/tau)^2) * sin(2*pi*f*n),
+ * evaluated in double, rounded to float once, n = 0..nsteps-1. */
+int gd_gaussian_pulse_table(size_t nsteps, double t0, double tau, double f_cps, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDIAMOND_GD_CAPI_H */

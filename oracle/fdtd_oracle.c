@@ -1,0 +1,323 @@
+/*
+ * CPU oracle for the 3D Yee FDTD time-stepping path -- TEST INFRASTRUCTURE.
+ *
+ * Only the test-suite, the package smoke check and the benchmark's CPU-baseline
+ * leg load this library, always as the checker. The product never links it.
+ *
+ * Every function restates the reference algorithm it names; pinning of each
+ * function (golden vectors from the reference build, closed forms, invariants)
+ * lives in tests/test_oracle_*.py.
+ *
+ * Build: gcc -O3 -fopenmp -ffp-contract=off (x86-64 baseline: no FMA), so each
+ * float operation rounds exactly once, as the reference CPU build does
+ * (CMakeLists.txt:23) and as its CUDA build forces with --fmad=false
+ * (CMakeLists.txt:25).
+ */
+#include "fdtd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI_F 3.14159265359f /* utils.h:17 */
+#define OR_PI_D 3.14159265358979323846
+
+/* ------------------------------------------------------------------ */
+/* point sources                                                       */
+/* ------------------------------------------------------------------ */
+
+/* The reference keeps full J/M arrays that are zero except at the source
+ * (gdiamond.hpp:1184-1191) and always subtracts J[idx]*dx. For every cell
+ * without a source that term is (+0)*dx = +0 and x - (+0) == x bit for bit,
+ * so applying the term only at source cells is exact. or_run_f32_jm_arrays
+ * keeps the literal array form to pin this. */
+static inline float src_val_f(int nsrc, const or_source* src, int comp,
+                              int64_t i, int64_t j, int64_t k, int64_t n) {
+  float v = 0.0f;
+  for (int s = 0; s < nsrc; ++s)
+    if (src[s].comp == comp && src[s].i == i && src[s].j == j && src[s].k == k)
+      v = src[s].table[n];
+  return v;
+}
+
+static int64_t plane_size(const or_dft* d, int64_t nx, int64_t ny, int64_t nz) {
+  if (d->axis == 0) return ny * nz;
+  if (d->axis == 1) return nx * nz;
+  return nx * ny;
+}
+
+void or_dft_weights(double f_cps, int64_t n, double scale, double* wc, double* ws) {
+  const double ang = 2.0 * OR_PI_D * f_cps * (double)n;
+  *wc = cos(ang) * scale;
+  *ws = sin(ang) * scale;
+}
+
+/* kernels.cuh:87-113 convention, FP64 accumulators. */
+#define DEFINE_DFT(NAME, T)                                                             \
+  static void NAME(or_dft* d, int64_t nx, int64_t ny, int64_t nz, T* const f[6],        \
+                   int64_t n) {                                                          \
+    if (!d || d->ncomp <= 0) return;                                                     \
+    const int64_t P = plane_size(d, nx, ny, nz);                                         \
+    const int64_t sxy = nx * ny;                                                         \
+    for (int c = 0; c < d->ncomp; ++c) {                                                 \
+      const T* a = f[d->comps[c]];                                                       \
+      for (int q = 0; q < d->nf; ++q) {                                                  \
+        double wc, ws;                                                                   \
+        or_dft_weights(d->f_cps[q], n, d->scale, &wc, &ws);                              \
+        double* re = d->re + ((int64_t)c * d->nf + q) * P;                               \
+        double* im = d->im + ((int64_t)c * d->nf + q) * P;                               \
+        if (d->axis == 2) {                                                              \
+          const T* pl = a + d->index * sxy;                                              \
+          for (int64_t p = 0; p < P; ++p) {                                              \
+            re[p] += (double)pl[p] * wc;                                                 \
+            im[p] -= (double)pl[p] * ws;                                                 \
+          }                                                                              \
+        } else if (d->axis == 1) {                                                       \
+          for (int64_t k = 0; k < nz; ++k)                                               \
+            for (int64_t i = 0; i < nx; ++i) {                                           \
+              const double e = (double)a[i + d->index * nx + k * sxy];                   \
+              re[i + k * nx] += e * wc;                                                  \
+              im[i + k * nx] -= e * ws;                                                  \
+            }                                                                            \
+        } else {                                                                         \
+          for (int64_t k = 0; k < nz; ++k)                                               \
+            for (int64_t j = 0; j < ny; ++j) {                                           \
+              const double e = (double)a[d->index + j * nx + k * sxy];                   \
+              re[j + k * ny] += e * wc;                                                  \
+              im[j + k * ny] -= e * ws;                                                  \
+            }                                                                            \
+        }                                                                                \
+      }                                                                                  \
+    }                                                                                    \
+  }
+
+DEFINE_DFT(dft_acc_f32, float)
+DEFINE_DFT(dft_acc_f64, double)
+
+/* ------------------------------------------------------------------ */
+/* stepping, gdiamond.hpp:1164-1221 (seq) / gdiamond_omp.hpp:110-159   */
+/* ------------------------------------------------------------------ */
+
+/* One body generator for float and double. Operand order is the
+ * reference's: Ca*E + Cb*(((a - b) - (c - d)) - J*dx). */
+#define DEFINE_RUN(NAME, T, DFTACC)                                                      \
+  int NAME(int64_t nx, int64_t ny, int64_t nz, T dx, T* const F[6],                      \
+           const or_material* mat, int nsrc, const or_source* src, int64_t n0,           \
+           int64_t nsteps, or_dft* dft, int nthreads) {                                  \
+    if (nx < 3 || ny < 3 || nz < 3) return -1;                                           \
+    T* Ex = F[0]; T* Ey = F[1]; T* Ez = F[2];                                            \
+    T* Hx = F[3]; T* Hy = F[4]; T* Hz = F[5];                                            \
+    const int64_t sxy = nx * ny;                                                         \
+    const float* const* C = mat->c;                                                      \
+    const int full = mat->full;                                                          \
+    const float* cb = mat->cb;                                                           \
+    const T db = (T)mat->db;                                                             \
+    int nth = nthreads < 1 ? 1 : nthreads;                                               \
+    for (int64_t n = n0; n < n0 + nsteps; ++n) {                                         \
+      /* update E */                                                                     \
+      _Pragma("omp parallel for collapse(2) schedule(static) num_threads(nth) if(nth > 1)") \
+      for (int64_t k = 1; k < nz - 1; k++) {                                             \
+        for (int64_t j = 1; j < ny - 1; j++) {                                           \
+          for (int64_t i = 1; i < nx - 1; i++) {                                         \
+            const int64_t idx = i + j * nx + k * sxy;                                    \
+            T jx = 0, jy = 0, jz = 0;                                                    \
+            if (nsrc) {                                                                  \
+              jx = (T)src_val_f(nsrc, src, 0, i, j, k, n);                               \
+              jy = (T)src_val_f(nsrc, src, 1, i, j, k, n);                               \
+              jz = (T)src_val_f(nsrc, src, 2, i, j, k, n);                               \
+            }                                                                            \
+            const T cx = ((Hz[idx] - Hz[idx - nx]) - (Hy[idx] - Hy[idx - sxy])) - jx * dx; \
+            const T cy = ((Hx[idx] - Hx[idx - sxy]) - (Hz[idx] - Hz[idx - 1])) - jy * dx;  \
+            const T cz = ((Hy[idx] - Hy[idx - 1]) - (Hx[idx] - Hx[idx - nx])) - jz * dx;   \
+            if (full) {                                                                  \
+              Ex[idx] = (T)C[0][idx] * Ex[idx] + (T)C[1][idx] * cx;                      \
+              Ey[idx] = (T)C[2][idx] * Ey[idx] + (T)C[3][idx] * cy;                      \
+              Ez[idx] = (T)C[4][idx] * Ez[idx] + (T)C[5][idx] * cz;                      \
+            } else {                                                                     \
+              const T b = (T)cb[idx];                                                    \
+              Ex[idx] = Ex[idx] + b * cx;                                                \
+              Ey[idx] = Ey[idx] + b * cy;                                                \
+              Ez[idx] = Ez[idx] + b * cz;                                                \
+            }                                                                            \
+          }                                                                              \
+        }                                                                                \
+      }                                                                                  \
+      /* update H */                                                                     \
+      _Pragma("omp parallel for collapse(2) schedule(static) num_threads(nth) if(nth > 1)") \
+      for (int64_t k = 1; k < nz - 1; k++) {                                             \
+        for (int64_t j = 1; j < ny - 1; j++) {                                           \
+          for (int64_t i = 1; i < nx - 1; i++) {                                         \
+            const int64_t idx = i + j * nx + k * sxy;                                    \
+            T mx = 0, my = 0, mz = 0;                                                    \
+            if (nsrc) {                                                                  \
+              mx = (T)src_val_f(nsrc, src, 3, i, j, k, n);                               \
+              my = (T)src_val_f(nsrc, src, 4, i, j, k, n);                               \
+              mz = (T)src_val_f(nsrc, src, 5, i, j, k, n);                               \
+            }                                                                            \
+            const T cx = ((Ey[idx + sxy] - Ey[idx]) - (Ez[idx + nx] - Ez[idx])) - mx * dx; \
+            const T cy = ((Ez[idx + 1] - Ez[idx]) - (Ex[idx + sxy] - Ex[idx])) - my * dx;  \
+            const T cz = ((Ex[idx + nx] - Ex[idx]) - (Ey[idx + 1] - Ey[idx])) - mz * dx;   \
+            if (full) {                                                                  \
+              Hx[idx] = (T)C[6][idx] * Hx[idx] + (T)C[7][idx] * cx;                      \
+              Hy[idx] = (T)C[8][idx] * Hy[idx] + (T)C[9][idx] * cy;                      \
+              Hz[idx] = (T)C[10][idx] * Hz[idx] + (T)C[11][idx] * cz;                    \
+            } else {                                                                     \
+              Hx[idx] = Hx[idx] + db * cx;                                               \
+              Hy[idx] = Hy[idx] + db * cy;                                               \
+              Hz[idx] = Hz[idx] + db * cz;                                               \
+            }                                                                            \
+          }                                                                              \
+        }                                                                                \
+      }                                                                                  \
+      DFTACC(dft, nx, ny, nz, F, n);                                                     \
+    }                                                                                    \
+    return 0;                                                                            \
+  }
+
+DEFINE_RUN(or_run_f32, float, dft_acc_f32)
+DEFINE_RUN(or_run_f64, double, dft_acc_f64)
+
+/* Literal reference loop with J/M arrays (gdiamond.hpp:1164-1221): the arrays
+ * are zero, each step writes the source values at their cells and every cell
+ * subtracts J[idx]*dx. */
+int or_run_f32_jm_arrays(int64_t nx, int64_t ny, int64_t nz, float dx,
+                         float* const F[6], const float* const C[12],
+                         int nsrc, const or_source* src, int64_t n0, int64_t nsteps) {
+  const int64_t N = nx * ny * nz, sxy = nx * ny;
+  float* JM[6];
+  for (int c = 0; c < 6; ++c) {
+    JM[c] = (float*)calloc((size_t)N, sizeof(float));
+    if (!JM[c]) return -2;
+  }
+  float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
+  float *Jx = JM[0], *Jy = JM[1], *Jz = JM[2], *Mx = JM[3], *My = JM[4], *Mz = JM[5];
+  for (int64_t t = n0; t < n0 + nsteps; t++) {
+    for (int s = 0; s < nsrc; ++s)
+      JM[src[s].comp][src[s].i + src[s].j * nx + src[s].k * sxy] = src[s].table[t];
+    for (int64_t k = 1; k < nz - 1; k++)
+      for (int64_t j = 1; j < ny - 1; j++)
+        for (int64_t i = 1; i < nx - 1; i++) {
+          int64_t idx = i + j * nx + k * sxy;
+          Ex[idx] = C[0][idx] * Ex[idx] + C[1][idx] *
+            ((Hz[idx] - Hz[idx - nx]) - (Hy[idx] - Hy[idx - sxy]) - Jx[idx] * dx);
+          Ey[idx] = C[2][idx] * Ey[idx] + C[3][idx] *
+            ((Hx[idx] - Hx[idx - sxy]) - (Hz[idx] - Hz[idx - 1]) - Jy[idx] * dx);
+          Ez[idx] = C[4][idx] * Ez[idx] + C[5][idx] *
+            ((Hy[idx] - Hy[idx - 1]) - (Hx[idx] - Hx[idx - nx]) - Jz[idx] * dx);
+        }
+    for (int64_t k = 1; k < nz - 1; k++)
+      for (int64_t j = 1; j < ny - 1; j++)
+        for (int64_t i = 1; i < nx - 1; i++) {
+          int64_t idx = i + j * nx + k * sxy;
+          Hx[idx] = C[6][idx] * Hx[idx] + C[7][idx] *
+            ((Ey[idx + sxy] - Ey[idx]) - (Ez[idx + nx] - Ez[idx]) - Mx[idx] * dx);
+          Hy[idx] = C[8][idx] * Hy[idx] + C[9][idx] *
+            ((Ez[idx + 1] - Ez[idx]) - (Ex[idx + sxy] - Ex[idx]) - My[idx] * dx);
+          Hz[idx] = C[10][idx] * Hz[idx] + C[11][idx] *
+            ((Ex[idx + nx] - Ex[idx]) - (Ey[idx + 1] - Ey[idx]) - Mz[idx] * dx);
+        }
+  }
+  for (int c = 0; c < 6; ++c) free(JM[c]);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* default problem, gdiamond.hpp:20-73 + utils.h:542-690               */
+/* ------------------------------------------------------------------ */
+
+/* Complex float arithmetic of utils.h:31-116 (each product/quotient in float). */
+typedef struct { float re, im; } cpx;
+static inline cpx cmul(cpx a, cpx b) {
+  cpx r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.re * b.im + a.im * b.re;
+  return r;
+}
+static inline cpx cdiv(cpx a, cpx b) {
+  const float den = b.re * b.re + b.im * b.im;
+  cpx r;
+  r.re = (a.re * b.re + a.im * b.im) / den;
+  r.im = (a.im * b.re - a.re * b.im) / den;
+  return r;
+}
+
+/* Physical constants of the default problem. eps0 = mu0 = 1 (utils.h:19-22);
+ * dx = wavelength/10 = 0.1f, dt = 0.05f, omega = 2*PI*c0/lambda in float
+ * (gdiamond.hpp:894-906). */
+static float def_dx(void) { return 1.0f / 10; }
+static float def_dt(void) { return 0.05f; }
+static float def_omega(void) {
+  const float c0 = 1.0f / sqrtf(1.0f * 1.0f);
+  const float freq = c0 / 1.0f;
+  return 2 * OR_PI_F * freq;
+}
+
+/* Grading of one axis: distance into the absorbing layer (utils.h:609-618). */
+static float bound_dist(int i, int N, int t_pml) {
+  float d = 0;
+  if (i < t_pml) d = (float)(1.0 - (double)((float)i / (float)t_pml));
+  if (i + t_pml >= N) d = (float)(1.0 - (double)((float)(N - i - 1) / (float)t_pml));
+  return d;
+}
+
+void or_default_problem_coefficients(int64_t nx64, int64_t ny64, int64_t nz64, float* const C[12]) {
+  const int Nx = (int)nx64, Ny = (int)ny64, Nz = (int)nz64;
+  const float dx = def_dx(), dt = def_dt(), omega0 = def_omega();
+  const float eps0 = 1.0f, mu0 = 1.0f;
+  const float eta0 = sqrtf(mu0 / eps0);
+  /* t_PML = ceil(1 um / dx) (gdiamond.hpp:63-64) */
+  const int t_pml = (int)ceilf(1.0f / dx);
+  const float a_max = 2.0f;
+  const int p = 3;
+  const float den = 2 * eta0 * (float)t_pml * dx;
+  const float sigma_max = (float)((double)(-(p + 1)) * log(1e-5) / (double)den);
+  for (int i = 0; i < Nx; ++i)
+    for (int j = 0; j < Ny; ++j)
+      for (int k = 0; k < Nz; ++k) {
+        /* the default mask is all-false: eps_r = eps_air = 1 + 0i everywhere */
+        cpx eps_r = {1.0f, 0.0f};
+        cpx mu_r = {1.0f, 0.0f};
+        const int64_t idx = (int64_t)i + (int64_t)j * Nx + (int64_t)k * Nx * Ny;
+        cpx s[3];
+        const int ijk[3] = {i, j, k}, NN[3] = {Nx, Ny, Nz};
+        for (int a = 0; a < 3; ++a) {
+          const float d = bound_dist(ijk[a], NN[a], t_pml);
+          const double pw = pow((double)d, (double)p);
+          s[a].re = (float)(1 + (double)a_max * pw);
+          s[a].im = (float)((double)sigma_max * pw / (double)(omega0 * eps0));
+        }
+        /* eps_xx = eps*sy*sz/sx etc. (utils.h:646-653) */
+        cpx e[3], m[3];
+        e[0] = cdiv(cmul(cmul(eps_r, s[1]), s[2]), s[0]);
+        e[1] = cdiv(cmul(cmul(eps_r, s[0]), s[2]), s[1]);
+        e[2] = cdiv(cmul(cmul(eps_r, s[0]), s[1]), s[2]);
+        m[0] = cdiv(cmul(cmul(mu_r, s[1]), s[2]), s[0]);
+        m[1] = cdiv(cmul(cmul(mu_r, s[0]), s[2]), s[1]);
+        m[2] = cdiv(cmul(cmul(mu_r, s[0]), s[1]), s[2]);
+        for (int a = 0; a < 3; ++a) {
+          float er = e[a].re; if (er < 1) er = 1;
+          float mr = m[a].re; if (mr < 1) mr = 1;
+          const float se = fabsf(omega0 * eps0 * e[a].im);
+          const float sh = fabsf(omega0 * mu0 * m[a].im);
+          float tmp = (float)((double)(se * dt) / (2.0 * (double)er * (double)eps0));
+          C[2 * a][idx] = (float)((1.0 - (double)tmp) / (1.0 + (double)tmp));
+          C[2 * a + 1][idx] = (float)((double)(dt / (er * eps0)) / (1.0 + (double)tmp) / (double)dx);
+          tmp = (float)((double)(sh * dt) / (2.0 * (double)mr * (double)mu0));
+          C[6 + 2 * a][idx] = (float)((1.0 - (double)tmp) / (1.0 + (double)tmp));
+          C[6 + 2 * a + 1][idx] = (float)((double)(dt / (mr * mu0)) / (1.0 + (double)tmp) / (double)dx);
+        }
+      }
+}
+
+float or_default_mz(int64_t t) {
+  /* M_source_amp = J_source_amp * pow(eta0, 3.0) = 5e4 (float member) */
+  const float m_amp = (float)(5e4f * pow((double)sqrtf(1.0f / 1.0f), 3.0));
+  return m_amp * sinf(def_omega() * (float)t * def_dt());
+}
+
+int64_t or_default_source_idx(int64_t nx, int64_t ny, int64_t nz) {
+  return nx / 2 + ny / 2 * nx + nz / 2 * nx * ny;
+}

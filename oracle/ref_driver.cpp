@@ -1,0 +1,58 @@
+// Thin C entry points over the reference's own CPU path, compiled from the
+// reference headers where they lie (see oracle/Makefile, target `ref`).
+// TEST INFRASTRUCTURE ONLY: used in this container to write golden vectors
+// and to cross-check the oracle restatement. Nothing here is shipped or run
+// by the product.
+//
+// The reference keeps its arrays private (gdiamond.hpp:798-891); the standard
+// headers are included first and `private` is widened only for the one
+// reference header so the results can be copied out unchanged.
+#include <stdio.h>
+#include <stdlib.h>
+#include <math.h>
+#include <string.h>
+#include <chrono>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#define private public
+#include "gdiamond.hpp"
+#undef private
+
+extern "C" {
+
+// gDiamond(Nx,Ny,Nz) (gdiamond.hpp:20-73): copy the 12 coefficient arrays
+// Cax Cbx Cay Cby Caz Cbz Dax Dbx Day Dby Daz Dbz into out[12*N].
+int ref_default_coefficients(int nx, int ny, int nz, float* out) {
+  gdiamond::gDiamond g(nx, ny, nz);
+  const size_t N = (size_t)nx * ny * nz;
+  const std::vector<float>* c[12] = {&g._Cax, &g._Cbx, &g._Cay, &g._Cby, &g._Caz, &g._Cbz,
+                                     &g._Dax, &g._Dbx, &g._Day, &g._Dby, &g._Daz, &g._Dbz};
+  for (int a = 0; a < 12; ++a) memcpy(out + a * N, c[a]->data(), N * sizeof(float));
+  return 0;
+}
+
+// update_FDTD_seq(T) (gdiamond.hpp:1164-1263): copy _Ex_seq.._Hz_seq into out[6*N].
+int ref_seq(int nx, int ny, int nz, int T, float* out) {
+  gdiamond::gDiamond g(nx, ny, nz);
+  g.update_FDTD_seq((size_t)T);
+  const size_t N = (size_t)nx * ny * nz;
+  const std::vector<float>* f[6] = {&g._Ex_seq, &g._Ey_seq, &g._Ez_seq,
+                                    &g._Hx_seq, &g._Hy_seq, &g._Hz_seq};
+  for (int a = 0; a < 6; ++a) memcpy(out + a * N, f[a]->data(), N * sizeof(float));
+  return 0;
+}
+
+// The per-step source value update_FDTD_seq writes (gdiamond.hpp:1189-1191),
+// evaluated by the reference object's own members.
+float ref_default_mz(int t) {
+  gdiamond::gDiamond g(3, 3, 3);
+  return g.M_source_amp * std::sin(g.SOURCE_OMEGA * (size_t)t * g.dt);
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+"""Shared problem builders for the tests (inputs only; identical arrays go to
+the HIP library and to the oracle)."""
+import numpy as np
+
+DT_C1 = np.float32(0.99 / np.sqrt(3.0))  # Courant 0.99 at dx = 1
+
+
+def iso_problem(gd, shape, seed=7, nspheres=12, rmin=2.0, rmax=6.0, emin=1.0, emax=12.0,
+                dt=DT_C1, dx=1.0):
+    """Sphere medium -> (cb, db) in the library's isotropic form."""
+    nx, ny, nz = shape
+    eps = gd.sphere_medium(nx, ny, nz, seed, nspheres, rmin, rmax, emin, emax)
+    cb, db = gd.iso_cb_from_eps(eps, dt, dx)
+    return eps, cb, db
+
+
+def centre(shape):
+    nx, ny, nz = shape
+    return nx // 2, ny // 2, nz // 2
+
+
+def random_fields(shape, seed, scale=1.0, boundary_zero=False):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    out = [(rng.standard_normal((nz, ny, nx)) * scale).astype(np.float32) for _ in range(6)]
+    if boundary_zero:
+        for f in out:
+            f[0], f[-1] = 0, 0
+            f[:, 0], f[:, -1] = 0, 0
+            f[:, :, 0], f[:, :, -1] = 0, 0
+    return out
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def first_diff(a, b):
+    d = np.argwhere(np.ascontiguousarray(a).view(np.uint32) != np.ascontiguousarray(b).view(np.uint32))
+    if d.size == 0:
+        return None
+    k, j, i = d[0]
+    return (int(i), int(j), int(k)), float(a[k, j, i]), float(b[k, j, i]), int(d.shape[0])
+
+
+def rel_err(a, b):
+    den = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a.astype(np.float64) - b))) / den if den > 0 else float(
+        np.max(np.abs(a)))

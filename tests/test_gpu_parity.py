@@ -1,0 +1,240 @@
+"""HIP path vs the oracle and the reference's golden vectors (GPU).
+
+The reference's own known-answer relation is seq == GPU to |a-b| < 1e-8 on
+every element (unittests/checker_gpu.cu:9-22, gdiamond.hpp:1330-1366). Both
+kernels here use the reference operand order with one rounding per float
+operation, so the bar is bit equality.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cases import DT_C1, bits_equal, centre, first_diff, iso_problem, random_fields, rel_err
+from conftest import golden_cases, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TILINGS = [
+    dict(blt=1, rows=2, waves=4, zchunk=7),
+    dict(blt=2, rows=2, waves=8, zchunk=5),
+    dict(blt=3, rows=4, waves=8, zchunk=64),
+    dict(blt=4, rows=4, waves=4, zchunk=9),
+    dict(blt=3, rows=2, waves=4, zchunk=3),
+    dict(blt=4, rows=2, waves=8, zchunk=16),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_fields_equal(got, want, label):
+    for c in range(6):
+        if not bits_equal(got[c], want[c]):
+            fd = first_diff(got[c], want[c])
+            raise AssertionError(f"{label}: component {c} differs: first (i,j,k)={fd[0]} "
+                                 f"got={fd[1]} want={fd[2]} ndiff={fd[3]} "
+                                 f"rel={rel_err(got[c], want[c]):.3e}")
+
+
+def run_variant(gd, shape, setup, nsteps, variant, tiling=None, chunks=None):
+    ctx = gd.Context(*shape, dx=setup.get("dx", 1.0))
+    try:
+        if "coeffs" in setup:
+            ctx.set_coefficients(setup["coeffs"])
+        else:
+            ctx.set_isotropic(setup["cb"], setup["db"])
+        for s in setup.get("sources", ()):
+            ctx.add_source(*s)
+        if "fields" in setup:
+            ctx.set_fields(setup["fields"])
+        if "dft" in setup:
+            d = setup["dft"]
+            ctx.dft_config(d["comps"], d["axis"], d["index"], d["f_cps"], d.get("scale", 1.0))
+        for n in (chunks or [nsteps]):
+            ctx.run(n, variant, tiling)
+        out = ctx.get_fields()
+        dft = None
+        if "dft" in setup:
+            dft = {c: ctx.dft_get(c) for c in setup["dft"]["comps"]}
+        return out, dft
+    finally:
+        ctx.close()
+
+
+def oracle_run(O, shape, setup, nsteps, threads=8):
+    f = [a.copy() for a in setup["fields"]] if "fields" in setup else O.zero_fields(shape)
+    kw = {}
+    if "coeffs" in setup:
+        kw["coeffs"] = setup["coeffs"]
+    else:
+        kw["cb"], kw["db"] = setup["cb"], setup["db"]
+    dft = None
+    if "dft" in setup:
+        dft = dict(setup["dft"])
+    res = O.run(shape, f, dx=np.float32(setup.get("dx", 1.0)), sources=setup.get("sources", ()),
+                nsteps=nsteps, dft=dft, threads=threads, **kw)
+    return f, res
+
+
+# ------------------------------------------------------------------ golden vectors
+
+
+@pytest.mark.parametrize("name", golden_cases())
+@pytest.mark.parametrize("variant", ["plain", "tiled3", "tiled4"])
+def test_reference_default_problem_golden(gpu, name, variant):
+    """gDiamond(nx,ny,nz).update_FDTD_seq(T) of the reference, bit for bit."""
+    g = load_golden(name)
+    nx, ny, nz = (int(v) for v in g["shape_xyz"])
+    T = int(g["steps"])
+    coeffs = [g[n] for n in gpu.COEFF_NAMES]
+    i, j, k = nx // 2, ny // 2, nz // 2
+    setup = dict(coeffs=coeffs, dx=np.float32(0.1), sources=[(gpu.HZ, i, j, k, g["Mz_table"])])
+    say(f"[golden] {name} {variant}")
+    if variant == "plain":
+        got, _ = run_variant(gpu, (nx, ny, nz), setup, T, gpu.PLAIN)
+    else:
+        t = (dict(blt=3, rows=2, waves=4, zchunk=5) if variant == "tiled3"
+             else dict(blt=4, rows=4, waves=4, zchunk=6))
+        got, _ = run_variant(gpu, (nx, ny, nz), setup, T, gpu.TILED, t)
+    assert_fields_equal(got, [g[c] for c in gpu.COMPONENTS], f"{name}/{variant}")
+
+
+# ------------------------------------------------------------------ oracle parity
+
+
+@pytest.mark.parametrize("shape", [(37, 29, 23), (70, 45, 33), (64, 32, 20), (5, 4, 3)])
+def test_iso_sphere_medium_plain_and_tiled(gpu, oracle, shape):
+    """Isotropic sphere medium + Gaussian Jz point source, every tiling, bit-exact."""
+    eps, cb, db = iso_problem(gpu, shape, seed=11, nspheres=6, rmin=2, rmax=5)
+    T = 23
+    src = gpu.gaussian_pulse_table(T, 8.0, 3.0, 0.07)
+    i, j, k = centre(shape)
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, i, j, k, src), (gpu.HX, 1, 1, 1, src * 0.5)])
+    want, _ = oracle_run(oracle, shape, setup, T)
+    say(f"[iso] {shape} plain")
+    got, _ = run_variant(gpu, shape, setup, T, gpu.PLAIN)
+    assert_fields_equal(got, want, f"plain {shape}")
+    for t in TILINGS:
+        say(f"[iso] {shape} tiled {t}")
+        got, _ = run_variant(gpu, shape, setup, T, gpu.TILED, t)
+        assert_fields_equal(got, want, f"tiled {t} {shape}")
+
+
+def test_random_initial_fields_full_material(gpu, oracle):
+    """Random fields (boundary included) + random full coefficient arrays."""
+    shape = (41, 38, 27)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(5)
+    coeffs = []
+    for a in range(12):
+        if a % 2 == 0:
+            coeffs.append(rng.uniform(0.9, 1.0, (nz, ny, nx)).astype(np.float32))
+        else:
+            coeffs.append(rng.uniform(0.05, 0.5, (nz, ny, nx)).astype(np.float32))
+    fields = random_fields(shape, 9)
+    setup = dict(coeffs=coeffs, fields=fields, dx=np.float32(0.7))
+    T = 9
+    want, _ = oracle_run(oracle, shape, setup, T)
+    got, _ = run_variant(gpu, shape, setup, T, gpu.PLAIN)
+    assert_fields_equal(got, want, "plain full random")
+    for t in TILINGS[:4]:
+        say(f"[full-random] tiled {t}")
+        got, _ = run_variant(gpu, shape, setup, T, gpu.TILED, t)
+        assert_fields_equal(got, want, f"tiled {t} full random")
+    # boundary cells never change (PEC walls: only 1..n-2 are updated)
+    for c in range(6):
+        for sl in (np.s_[0], np.s_[-1], np.s_[:, 0], np.s_[:, -1], np.s_[:, :, 0], np.s_[:, :, -1]):
+            assert bits_equal(got[c][sl], fields[c][sl])
+
+
+def test_runs_continue_across_calls(gpu, oracle):
+    """run(10); run(13) == run(23): source and DFT step indices are absolute."""
+    shape = (33, 31, 29)
+    eps, cb, db = iso_problem(gpu, shape, seed=3)
+    T = 23
+    src = gpu.gaussian_pulse_table(T, 6.0, 3.0, 0.05)
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)],
+                 dft=dict(comps=[0, 2, 4], axis=2, index=15, f_cps=np.array([0.03, 0.05, 0.07])))
+    want, wdft = oracle_run(oracle, shape, setup, T)
+    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, dict(blt=4, rows=2, waves=4, zchunk=8))):
+        got, gdft = run_variant(gpu, shape, setup, T, variant, t, chunks=[10, 13])
+        assert_fields_equal(got, want, f"chunks variant {variant}")
+        for s, c in enumerate([0, 2, 4]):
+            assert bits_equal(gdft[c][0], wdft[0][s]) and bits_equal(gdft[c][1], wdft[1][s])
+
+
+@pytest.mark.parametrize("axis,index", [(2, 11), (0, 7), (1, 20), (2, 0)])
+def test_dft_monitor_fp64(gpu, oracle, axis, index):
+    """FP64 running DFT (kernels.cuh:87-113 convention) on every plane orientation."""
+    shape = (30, 26, 22)
+    eps, cb, db = iso_problem(gpu, shape, seed=21)
+    T = 31
+    src = gpu.gaussian_pulse_table(T, 10.0, 4.0, 0.05)
+    comps = [0, 1, 2, 3, 4, 5]
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)],
+                 dft=dict(comps=comps, axis=axis, index=index,
+                          f_cps=np.array([0.02, 0.05, 0.11]), scale=0.5))
+    want, (wre, wim) = oracle_run(oracle, shape, setup, T)
+    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, dict(blt=3, rows=2, waves=4, zchunk=6)),
+                       (gpu.TILED, dict(blt=2, rows=4, waves=8, zchunk=64))):
+        say(f"[dft] axis {axis} variant {variant} {t}")
+        got, gdft = run_variant(gpu, shape, setup, T, variant, t)
+        assert_fields_equal(got, want, "dft fields")
+        for s, c in enumerate(comps):
+            re, im = gdft[c]
+            assert bits_equal(re, wre[s]), f"re comp {c} maxdiff {np.abs(re - wre[s]).max()}"
+            assert bits_equal(im, wim[s]), f"im comp {c}"
+
+
+# ------------------------------------------------------------------ BASELINE configs
+
+
+def test_config_vacuum_128_200_steps(gpu, oracle):
+    """128^3 vacuum, 200 steps, Courant 0.99, Gaussian Jz at the centre, PEC walls:
+    all six fields vs the sequential CPU update (bar: max rel error <= 1e-5;
+    observed: bit equality)."""
+    shape = (128, 128, 128)
+    nx, ny, nz = shape
+    cb = np.full((nz, ny, nx), np.float32(np.float64(DT_C1) / 1.0), np.float32)
+    db = np.float32(np.float64(DT_C1) / 1.0)
+    T = 200
+    src = gpu.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)])
+    want, _ = oracle_run(oracle, shape, setup, T, threads=16)
+    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, None)):
+        say(f"[c1] variant {variant}")
+        got, _ = run_variant(gpu, shape, setup, T, variant, t)
+        for c in range(6):
+            assert rel_err(got[c], want[c]) <= 1e-5
+        assert_fields_equal(got, want, f"c1 variant {variant}")
+
+
+def test_config_spheres_512_tiled_vs_plain_and_oracle(gpu, oracle):
+    """Full-size 512^3 sphere medium (200 spheres, r 4-20, eps 1-12, seed 42):
+    6 steps vs the oracle (OpenMP) bit for bit, then 60 steps TILED vs PLAIN bit
+    for bit including the FP64 DFT on the z=256 plane."""
+    shape = (512, 512, 512)
+    eps, cb, db = iso_problem(gpu, shape, seed=42, nspheres=200, rmin=4, rmax=20, emin=1,
+                              emax=12)
+    del eps
+    T0 = 6
+    src = gpu.gaussian_pulse_table(1000, 30.0, 10.0, 0.05)
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)])
+    say("[512] oracle 6 steps")
+    want, _ = oracle_run(oracle, shape, setup, T0, threads=16)
+    say("[512] tiled 6 steps")
+    got, _ = run_variant(gpu, shape, setup, T0, gpu.TILED)
+    assert_fields_equal(got, want, "512 tiled vs oracle")
+    del want, got
+    setup["dft"] = dict(comps=[0, 2], axis=2, index=256, f_cps=np.array([0.03, 0.05, 0.07]))
+    T1 = 60
+    say("[512] plain 60 steps")
+    a, adft = run_variant(gpu, shape, setup, T1, gpu.PLAIN)
+    say("[512] tiled 60 steps")
+    b, bdft = run_variant(gpu, shape, setup, T1, gpu.TILED)
+    assert_fields_equal(b, a, "512 tiled vs plain")
+    for c in (0, 2):
+        assert bits_equal(bdft[c][0], adft[c][0]) and bits_equal(bdft[c][1], adft[c][1])
