@@ -86,6 +86,12 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// DFT weights use separate cos() and sin() calls that the compiler may not
+// fuse into sincos(): glibc's sincos() differs from them in the last bit for
+// some arguments, and the weights are part of the bit-exact contract.
+__attribute__((noinline)) double weight_cos(double a) { return std::cos(a); }
+__attribute__((noinline)) double weight_sin(double a) { return std::sin(a); }
+
 int alloc_fields(gd_ctx* c, int copy) {
   for (int a = 0; a < 6; ++a) {
     if (c->f[copy][a]) continue;
@@ -528,8 +534,8 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     for (size_t q = 0; q < nf; ++q)
       for (size_t r = 0; r < nsteps; ++r) {
         const double ang = two_pi * c->dft_f[q] * (double)(c->step + r);
-        c->dft_w_host[q * nsteps + r] = std::cos(ang) * c->dft_scale;
-        c->dft_w_host[nf * nsteps + q * nsteps + r] = std::sin(ang) * c->dft_scale;
+        c->dft_w_host[q * nsteps + r] = weight_cos(ang) * c->dft_scale;
+        c->dft_w_host[nf * nsteps + q * nsteps + r] = weight_sin(ang) * c->dft_scale;
       }
     GD_HIP(c, hipMemcpyAsync(c->dft_w, c->dft_w_host.data(), need * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));

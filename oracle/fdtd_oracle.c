@@ -49,10 +49,15 @@ static int64_t plane_size(const or_dft* d, int64_t nx, int64_t ny, int64_t nz) {
   return nx * ny;
 }
 
+/* cos and sin are called separately and never fused into sincos(): glibc's
+ * sincos() differs from sin()/cos() in the last bit for some arguments. */
+static __attribute__((noinline)) double or_cos(double a) { return cos(a); }
+static __attribute__((noinline)) double or_sin(double a) { return sin(a); }
+
 void or_dft_weights(double f_cps, int64_t n, double scale, double* wc, double* ws) {
   const double ang = 2.0 * OR_PI_D * f_cps * (double)n;
-  *wc = cos(ang) * scale;
-  *ws = sin(ang) * scale;
+  *wc = or_cos(ang) * scale;
+  *ws = or_sin(ang) * scale;
 }
 
 /* kernels.cuh:87-113 convention, FP64 accumulators. */

@@ -51,3 +51,18 @@ def rel_err(a, b):
     den = float(np.max(np.abs(b)))
     return float(np.max(np.abs(a.astype(np.float64) - b))) / den if den > 0 else float(
         np.max(np.abs(a)))
+
+
+def describe_diff(got, want):
+    """Short description of how two float arrays differ (for failure messages)."""
+    g = np.ascontiguousarray(got).ravel()
+    w = np.ascontiguousarray(want).ravel()
+    ib = g.view(np.uint64 if g.dtype == np.float64 else np.uint32)
+    wb = w.view(np.uint64 if w.dtype == np.float64 else np.uint32)
+    bad = np.flatnonzero(ib != wb)
+    if bad.size == 0:
+        return "equal"
+    vals_equal = np.flatnonzero(g[bad] != w[bad])
+    q = bad[vals_equal[0]] if vals_equal.size else bad[0]
+    return (f"{bad.size} bit-differences, {vals_equal.size} value-differences, "
+            f"max |d| {np.abs(g - w).max():.3e}, first at flat {q}: got {g[q]!r} want {w[q]!r}")

@@ -1,0 +1,135 @@
+"""C ABI of libgdiamond_hip.so without a GPU: the library loads, exports every
+symbol include/gdiamond/gd_capi.h declares, and its host-side problem builders
+match the reference (golden vectors) and their stated formulas."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cases import bits_equal
+from conftest import ROOT, golden_cases, load_golden
+
+HEADER = os.path.join(ROOT, "include", "gdiamond", "gd_capi.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declarations_are_exported(gd):
+    L = C.CDLL(gd.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header(gd):
+    assert set(gd.exported_symbols()) == set(declared_symbols())
+
+
+def test_version(gd):
+    assert "gfx950" in gd.version()
+
+
+@pytest.mark.parametrize("name", golden_cases() + ["ref_coeff_40x33x27.npz"])
+def test_library_default_coefficients_match_reference(gd, name):
+    """gd_default_problem_coefficients == gDiamond(nx,ny,nz) coefficient arrays."""
+    g = load_golden(name)
+    nx, ny, nz = (int(v) for v in g["shape_xyz"])
+    mine = gd.default_problem_coefficients(nx, ny, nz)
+    for a, nm in enumerate(gd.COEFF_NAMES):
+        assert bits_equal(mine[a], g[nm]), nm
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_library_default_mz_matches_reference(gd, name):
+    g = load_golden(name)
+    assert bits_equal(gd.default_mz_table(int(g["steps"])), g["Mz_table"])
+
+
+def test_gaussian_pulse_formula(gd):
+    T = 300
+    t = gd.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
+    want = np.array([np.float32(math.exp(-(((n - 30.0) / 10.0) ** 2)) *
+                                math.sin(2 * math.pi * 0.05 * n)) for n in range(T)],
+                    np.float32)
+    assert np.abs(t - want).max() <= 1e-7
+    assert 0.5 < float(np.abs(t).max()) <= 1.0
+
+
+def splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def test_sphere_medium_definition(gd):
+    """SplitMix64 draws (cx, cy, cz, r, eps) per sphere; later spheres win."""
+    nx, ny, nz = 40, 36, 30
+    eps = gd.sphere_medium(nx, ny, nz, 42, 5, 3.0, 7.0, 1.0, 12.0)
+    assert eps.shape == (nz, ny, nx)
+    s = 42
+    spheres = []
+    for _ in range(5):
+        vals = []
+        for _ in range(5):
+            s, z = splitmix64(s)
+            vals.append((z >> 11) * 2.0 ** -53)
+        cx, cy, cz = vals[0] * nx, vals[1] * ny, vals[2] * nz
+        r = 3.0 + vals[3] * 4.0
+        e = np.float32(1.0 + vals[4] * 11.0)
+        spheres.append((cx, cy, cz, r, e))
+    want = np.ones((nz, ny, nx), np.float32)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    for cx, cy, cz, r, e in spheres:
+        inside = (i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2 <= r * r
+        want[inside] = e
+    assert bits_equal(eps, want)
+    assert (eps > 1).sum() > 100
+
+
+def test_sphere_medium_reproducible(gd):
+    a = gd.sphere_medium(64, 48, 40, 7)
+    b = gd.sphere_medium(64, 48, 40, 7)
+    c = gd.sphere_medium(64, 48, 40, 8)
+    assert bits_equal(a, b) and not bits_equal(a, c)
+    assert a.min() >= 1.0 and a.max() <= 12.0
+
+
+def test_create_without_device_or_with_bad_sizes(gd):
+    h = C.c_void_p()
+    rc = gd.lib().gd_create(C.byref(h), 2, 10, 10, 1.0)
+    assert rc == -1 and b"at least 3" in gd.lib().gd_last_error(None)
+    if gd.device_count() == 0:
+        rc = gd.lib().gd_create(C.byref(h), 10, 10, 10, 1.0)
+        assert rc == -6 and h.value is None
+        with pytest.raises(gd.GdError):
+            gd.Context(8, 8, 8)
+
+
+def test_host_builders_reject_bad_args(gd):
+    L = gd.lib()
+    assert L.gd_sphere_medium(4, 4, 4, 1, 1, 5.0, 2.0, 1.0, 2.0,
+                              np.empty(64, np.float32).ctypes.data_as(gd._FP)) == -1
+    assert L.gd_gaussian_pulse_table(4, 0.0, 0.0, 0.1,
+                                     np.empty(4, np.float32).ctypes.data_as(gd._FP)) == -1
+    null12 = (gd._FP * 12)()
+    assert L.gd_default_problem_coefficients(4, 4, 4, null12) == -1
+
+
+def test_library_has_no_oracle_dependency(gd):
+    """The product library never links the checker."""
+    import subprocess
+    out = subprocess.run(["ldd", gd.LIB_PATH], capture_output=True, text=True).stdout
+    assert "liboracle" not in out and "gdref" not in out
+    src = "".join(open(os.path.join(ROOT, "gdiamond_amd", "csrc", f)).read()
+                  for f in os.listdir(os.path.join(ROOT, "gdiamond_amd", "csrc")))
+    assert "oracle" not in src.lower().replace("the oracle", "")

@@ -10,7 +10,8 @@ import sys
 import numpy as np
 import pytest
 
-from cases import DT_C1, bits_equal, centre, first_diff, iso_problem, random_fields, rel_err
+from cases import (DT_C1, bits_equal, centre, describe_diff, first_diff, iso_problem,
+                   random_fields, rel_err)
 from conftest import golden_cases, load_golden
 
 pytestmark = pytest.mark.gpu
@@ -159,11 +160,15 @@ def test_runs_continue_across_calls(gpu, oracle):
     setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)],
                  dft=dict(comps=[0, 2, 4], axis=2, index=15, f_cps=np.array([0.03, 0.05, 0.07])))
     want, wdft = oracle_run(oracle, shape, setup, T)
-    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, dict(blt=4, rows=2, waves=4, zchunk=8))):
+    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, dict(blt=4, rows=4, waves=4, zchunk=8)),
+                       (gpu.TILED, dict(blt=3, rows=2, waves=8, zchunk=7))):
         got, gdft = run_variant(gpu, shape, setup, T, variant, t, chunks=[10, 13])
         assert_fields_equal(got, want, f"chunks variant {variant}")
         for s, c in enumerate([0, 2, 4]):
-            assert bits_equal(gdft[c][0], wdft[0][s]) and bits_equal(gdft[c][1], wdft[1][s])
+            assert bits_equal(gdft[c][0], wdft[0][s]), \
+                f"variant {variant} comp {c} re: {describe_diff(gdft[c][0], wdft[0][s])}"
+            assert bits_equal(gdft[c][1], wdft[1][s]), \
+                f"variant {variant} comp {c} im: {describe_diff(gdft[c][1], wdft[1][s])}"
 
 
 @pytest.mark.parametrize("axis,index", [(2, 11), (0, 7), (1, 20), (2, 0)])
@@ -185,8 +190,8 @@ def test_dft_monitor_fp64(gpu, oracle, axis, index):
         assert_fields_equal(got, want, "dft fields")
         for s, c in enumerate(comps):
             re, im = gdft[c]
-            assert bits_equal(re, wre[s]), f"re comp {c} maxdiff {np.abs(re - wre[s]).max()}"
-            assert bits_equal(im, wim[s]), f"im comp {c}"
+            assert bits_equal(re, wre[s]), f"variant {variant} comp {c} re: {describe_diff(re, wre[s])}"
+            assert bits_equal(im, wim[s]), f"variant {variant} comp {c} im: {describe_diff(im, wim[s])}"
 
 
 # ------------------------------------------------------------------ BASELINE configs
@@ -237,4 +242,5 @@ def test_config_spheres_512_tiled_vs_plain_and_oracle(gpu, oracle):
     b, bdft = run_variant(gpu, shape, setup, T1, gpu.TILED)
     assert_fields_equal(b, a, "512 tiled vs plain")
     for c in (0, 2):
-        assert bits_equal(bdft[c][0], adft[c][0]) and bits_equal(bdft[c][1], adft[c][1])
+        assert bits_equal(bdft[c][0], adft[c][0]), describe_diff(bdft[c][0], adft[c][0])
+        assert bits_equal(bdft[c][1], adft[c][1]), describe_diff(bdft[c][1], adft[c][1])

@@ -1,0 +1,181 @@
+"""Mathematical pins of the oracle beyond the golden vectors.
+
+* Discrete energy of the lossless Yee scheme is conserved exactly in exact
+  arithmetic (the E and H curls are adjoint when boundary cells stay zero):
+  W^n = sum eps E^n.E^n + sum mu H^(n-1/2).H^(n+1/2). A wrong sign or index in
+  any curl term breaks it.
+* Discrete divergence: forward-difference div H and backward-difference
+  div(eps E) are constant in time away from sources and walls.
+* DFT convention (kernels.cuh:87-113): re += v cos(2 pi f n) s, im -= v sin(..) s
+  with v the value after step n, checked against a direct numpy sum over the
+  recorded time series.
+* FP32 and FP64 oracles agree to single-precision rounding.
+"""
+import numpy as np
+import pytest
+
+from cases import centre, random_fields
+
+
+def _interior_random(shape, seed):
+    f = random_fields(shape, seed, boundary_zero=True)
+    return [a.astype(np.float64) for a in f]
+
+
+def energy(f, f_prev_h, eps, mu):
+    E = sum(np.sum(eps * a * a) for a in f[:3])
+    H = sum(np.sum(mu * a * b) for a, b in zip(f[3:], f_prev_h))
+    return E + H
+
+
+@pytest.mark.parametrize("shape", [(12, 10, 9), (16, 7, 11)])
+def test_energy_conserved_lossless(oracle, shape):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(1.0, 4.0, (nz, ny, nx))
+    dt, dx = 0.4, 1.0
+    cb = (dt / (eps * dx)).astype(np.float32)
+    db = np.float32(dt / dx)
+    f = _interior_random(shape, 1)
+    # cb is float32; use exactly those values as eps for the energy weights
+    eps_eff = dt / (cb.astype(np.float64) * dx)
+    mu_eff = dt / (np.float64(db) * dx)
+    Ws = []
+    for n in range(40):
+        h_prev = [a.copy() for a in f[3:]]
+        oracle.run(shape, f, cb=cb, db=db, dx=dx, nsteps=1, n0=n, dtype=np.float64)
+        # E^(n+1) with H^(n+1/2) . H^(n+3/2): use E after the step and H before/after
+        Ws.append(sum(np.sum(eps_eff * a * a) for a in f[:3]) +
+                  sum(np.sum(mu_eff * a * b) for a, b in zip(h_prev, f[3:])))
+    Ws = np.array(Ws)
+    assert np.abs(Ws - Ws[0]).max() / abs(Ws[0]) < 1e-12
+
+
+def test_energy_breaks_with_wrong_sign(oracle):
+    """Plant a sign error in the H coefficient (db -> -db): energy is no longer
+    conserved, so the check above can fail."""
+    shape = (12, 10, 9)
+    nx, ny, nz = shape
+    cb = np.full((nz, ny, nx), 0.4, np.float32)
+    f = _interior_random(shape, 1)
+    Ws = []
+    for n in range(20):
+        h_prev = [a.copy() for a in f[3:]]
+        oracle.run(shape, f, cb=cb, db=-0.4, dx=1.0, nsteps=1, n0=n, dtype=np.float64)
+        Ws.append(sum(np.sum(a * a) for a in f[:3]) / 0.4 +
+                  sum(np.sum(a * b) for a, b in zip(h_prev, f[3:])) / -0.4)
+    Ws = np.array(Ws)
+    assert np.abs(Ws - Ws[0]).max() / abs(Ws[0]) > 1e-3
+
+
+def div_h(f):
+    """Forward-difference divergence: annihilates the forward-difference curl
+    that updates H (gdiamond.hpp:1209-1221)."""
+    Hx, Hy, Hz = f[3], f[4], f[5]
+    return ((Hx[:-1, :-1, 1:] - Hx[:-1, :-1, :-1]) + (Hy[:-1, 1:, :-1] - Hy[:-1, :-1, :-1]) +
+            (Hz[1:, :-1, :-1] - Hz[:-1, :-1, :-1]))
+
+
+def div_eps_e(f, eps):
+    """Backward-difference divergence of eps*E (E is updated with backward
+    differences of H, gdiamond.hpp:1194-1206); entry [k-1,j-1,i-1] is cell (i,j,k)."""
+    Ex, Ey, Ez = (eps * f[0], eps * f[1], eps * f[2])
+    return ((Ex[1:, 1:, 1:] - Ex[1:, 1:, :-1]) + (Ey[1:, 1:, 1:] - Ey[1:, :-1, 1:]) +
+            (Ez[1:, 1:, 1:] - Ez[:-1, 1:, 1:]))
+
+
+def test_divergence_constant_away_from_source(oracle):
+    shape = (15, 13, 11)
+    nx, ny, nz = shape
+    dt = 0.5
+    rng = np.random.default_rng(8)
+    eps = rng.uniform(1.0, 3.0, (nz, ny, nx))
+    cb = (dt / eps).astype(np.float32)
+    eps_eff = dt / cb.astype(np.float64)
+    f = _interior_random(shape, 2)
+    dh0 = div_h(f)
+    de0 = div_eps_e(f, eps_eff)
+    T = 25
+    i, j, k = centre(shape)
+    src = np.sin(np.arange(T) * 0.3).astype(np.float32)
+    oracle.run(shape, f, cb=cb, db=dt, dx=1.0, sources=[(2, i, j, k, src)], nsteps=T,
+               dtype=np.float64)
+    # only cells whose whole stencil is updated (frozen boundary cells are not
+    # part of the curl identity): drop one layer on each side
+    dh = (div_h(f) - dh0)[1:-1, 1:-1, 1:-1]
+    assert np.abs(dh).max() < 1e-10
+    de = (div_eps_e(f, eps_eff) - de0)[1:-1, 1:-1, 1:-1]  # entry [k-2,j-2,i-2] = cell (i,j,k)
+    # the z-directed current charges exactly the two cells it connects
+    assert np.abs(de[k - 2, j - 2, i - 2]) > 1e-3 and np.abs(de[k - 1, j - 2, i - 2]) > 1e-3
+    de[k - 2:k, j - 2, i - 2] = 0.0
+    assert np.abs(de).max() < 1e-10
+
+
+def test_dft_convention_direct_sum(oracle):
+    """Accumulated monitor == direct sum over the recorded plane values."""
+    shape = (14, 12, 10)
+    nx, ny, nz = shape
+    cb = np.full((nz, ny, nx), 0.5, np.float32)
+    T = 30
+    src = (np.exp(-((np.arange(T) - 8) / 3.0) ** 2) * np.sin(0.6 * np.arange(T))).astype(
+        np.float32)
+    i, j, k = centre(shape)
+    f_cps = np.array([0.05, 0.13])
+    scale = 0.25
+    f = oracle.zero_fields(shape)
+    re, im = oracle.run(shape, f, cb=cb, db=0.5, dx=1.0, sources=[(2, i, j, k, src)], nsteps=T,
+                        dft=dict(comps=[2, 3], axis=2, index=k, f_cps=f_cps, scale=scale))
+    g = oracle.zero_fields(shape)
+    series = []
+    for n in range(T):
+        oracle.run(shape, g, cb=cb, db=0.5, dx=1.0, sources=[(2, i, j, k, src)], n0=n, nsteps=1)
+        series.append([g[2][k].astype(np.float64).ravel(), g[3][k].astype(np.float64).ravel()])
+    series = np.array(series)  # (T, 2, plane)
+    n = np.arange(T)
+    for q, fq in enumerate(f_cps):
+        w_c = np.cos(2 * np.pi * fq * n) * scale
+        w_s = np.sin(2 * np.pi * fq * n) * scale
+        for s in range(2):
+            want_re = np.einsum("t,tp->p", w_c, series[:, s])
+            want_im = -np.einsum("t,tp->p", w_s, series[:, s])
+            tol = 1e-12 * (np.abs(series[:, s]).sum(0) + 1e-30)
+            assert np.all(np.abs(re[s, q] - want_re) <= tol)
+            assert np.all(np.abs(im[s, q] - want_im) <= tol)
+    assert np.abs(re).max() > 0
+
+
+def test_fp32_vs_fp64_oracle(oracle):
+    shape = (24, 22, 20)
+    nx, ny, nz = shape
+    dt = np.float32(0.99 / np.sqrt(3.0))
+    cb = np.full((nz, ny, nx), dt, np.float32)
+    T = 60
+    n = np.arange(T)
+    src = (np.exp(-((n - 30) / 10.0) ** 2) * np.sin(2 * np.pi * 0.05 * n)).astype(np.float32)
+    c = centre(shape)
+    a = oracle.zero_fields(shape)
+    b = oracle.zero_fields(shape, np.float64)
+    oracle.run(shape, a, cb=cb, db=dt, dx=1.0, sources=[(2, *c, src)], nsteps=T)
+    oracle.run(shape, b, cb=cb, db=dt, dx=1.0, sources=[(2, *c, src)], nsteps=T,
+               dtype=np.float64)
+    # Hz of a z-directed dipole is zero by symmetry: normalise by the E and H
+    # scales rather than per component
+    e_scale = max(np.abs(y).max() for y in b[:3])
+    h_scale = max(np.abs(y).max() for y in b[3:])
+    for c, (x, y) in enumerate(zip(a, b)):
+        scale = e_scale if c < 3 else h_scale
+        assert np.abs(x - y).max() / scale < 1e-5
+
+
+def test_dft_weights_are_separate_libm_cos_sin(oracle):
+    """Weights are libm cos(2 pi f n)*s and sin(2 pi f n)*s as separate calls
+    (glibc's fused sincos() differs from them in the last bit for ~0.1% of
+    arguments; the library evaluates them the same way)."""
+    import math
+    bad = 0
+    for f in (0.02, 0.03, 0.05, 0.07, 0.11, 0.0446):
+        for n in range(0, 3000, 7):
+            ang = 2.0 * 3.14159265358979323846 * f * float(n)
+            wc, ws = oracle.dft_weights(f, n, 1.0)
+            bad += (wc != math.cos(ang)) + (ws != math.sin(ang))
+    assert bad == 0
