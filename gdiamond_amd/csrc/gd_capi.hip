@@ -41,6 +41,8 @@ struct gd_ctx {
     size_t len;
   };
   std::vector<Src> src;
+  gd::SourceDesc* src_desc = nullptr;         // device copy of the descriptors
+  std::vector<gd::SourceDesc> src_desc_host;
 
   int dft_ncomp = 0;
   int dft_comp[6] = {0};
@@ -53,6 +55,7 @@ struct gd_ctx {
   double* dft_im = nullptr;
   double* dft_w = nullptr;
   size_t dft_w_cap = 0;
+  float* dft_snap = nullptr;  // TILED: [level][component][plane] plane values
   std::vector<double> dft_w_host;
 
   bool launch_timing = false;
@@ -127,18 +130,20 @@ hipEvent_t pool_event(gd_ctx* c, size_t i) {
   return c->ev_pool[i];
 }
 
-// Tiling planner: B = 3 time levels, 4 rows x 8 waves (64 x 32 input tile,
-// 58 x 26 owned), z chunk sized so the launch has >= ~1024 workgroups
-// (4 per CU on 256 CUs) without dropping below 16 owned planes.
+// Tiling planner: B = 4 time levels, 4 rows x 8 waves (64 x 32 input tile,
+// 56 x 24 owned), the fastest measured shape at 512^3 on MI355X. The z chunk
+// is at most 64 owned planes and shrinks (down to 16) until the launch has
+// ~1024 workgroups (4 per CU on 256 CUs).
 gd_tiling plan(const gd_ctx* c) {
   gd_tiling t;
-  t.blt = 3;
+  t.blt = 4;
   t.rows = 4;
   t.waves = 8;
   const size_t ox = 64 - 2 * t.blt, oy = (size_t)(t.rows * t.waves - 2 * t.blt);
   const size_t txy = ((c->nx + ox - 1) / ox) * ((c->ny + oy - 1) / oy);
   size_t zc = c->nz * txy / 1024;
   if (zc < 16) zc = 16;
+  if (zc > 64) zc = 64;
   if (zc > c->nz) zc = c->nz;
   t.zchunk = (int)zc;
   return t;
@@ -205,9 +210,11 @@ int gd_destroy(gd_ctx* c) {
       if (c->f[k][a]) (void)hipFree(c->f[k][a]);
   free_material(c);
   for (auto& s : c->src) (void)hipFree(s.table);
+  if (c->src_desc) (void)hipFree(c->src_desc);
   if (c->dft_re) (void)hipFree(c->dft_re);
   if (c->dft_im) (void)hipFree(c->dft_im);
   if (c->dft_w) (void)hipFree(c->dft_w);
+  if (c->dft_snap) (void)hipFree(c->dft_snap);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -371,7 +378,9 @@ int gd_dft_disable(gd_ctx* c) {
   GD_HIP(c, hipStreamSynchronize(c->stream));
   if (c->dft_re) (void)hipFree(c->dft_re);
   if (c->dft_im) (void)hipFree(c->dft_im);
+  if (c->dft_snap) (void)hipFree(c->dft_snap);
   c->dft_re = c->dft_im = nullptr;
+  c->dft_snap = nullptr;
   c->dft_ncomp = 0;
   c->dft_f.clear();
   c->dft_plane = 0;
@@ -489,6 +498,11 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
                                          "{2,4}x{4,8})");
     int rc = alloc_fields(c, 1);
     if (rc != GD_OK) return rc;
+    if (c->dft_ncomp && !c->dft_snap) {
+      const size_t n = (size_t)gd::kMaxBlt * 6 * c->dft_plane;
+      hipError_t e = hipMalloc(&c->dft_snap, n * sizeof(float));
+      if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(dft snapshot)");
+    }
   }
 
   gd::Grid g;
@@ -505,12 +519,23 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
 
   gd::Sources s{};
   s.n = (int)c->src.size();
-  for (int q = 0; q < s.n; ++q) {
-    s.comp[q] = c->src[q].comp;
-    s.i[q] = (int)c->src[q].i;
-    s.j[q] = (int)c->src[q].j;
-    s.k[q] = (int)c->src[q].k;
-    s.table[q] = c->src[q].table;
+  if (s.n) {
+    if (!c->src_desc) {
+      hipError_t e = hipMalloc(&c->src_desc, GD_MAX_SOURCES * sizeof(gd::SourceDesc));
+      if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(source descriptors)");
+    }
+    c->src_desc_host.resize(s.n);
+    for (int q = 0; q < s.n; ++q) {
+      gd::SourceDesc& sd = c->src_desc_host[q];
+      sd.comp = c->src[q].comp;
+      sd.i = (int)c->src[q].i;
+      sd.j = (int)c->src[q].j;
+      sd.k = (int)c->src[q].k;
+      sd.table = c->src[q].table;
+    }
+    GD_HIP(c, hipMemcpyAsync(c->src_desc, c->src_desc_host.data(), s.n * sizeof(gd::SourceDesc),
+                             hipMemcpyHostToDevice, c->stream));
+    s.d = c->src_desc;
   }
 
   gd::Dft d{};
@@ -539,7 +564,8 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       }
     GD_HIP(c, hipMemcpyAsync(c->dft_w, c->dft_w_host.data(), need * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
-    for (int q = 0; q < d.ncomp; ++q) d.comp[q] = c->dft_comp[q];
+    d.mask = 0u;
+    for (int q = 0; q < d.ncomp; ++q) d.mask |= 1u << c->dft_comp[q];
     d.axis = c->dft_axis;
     d.index = (int)c->dft_index;
     d.nf = (int)nf;
@@ -550,6 +576,7 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     d.ws = c->dft_w + nf * nsteps;
     d.re = c->dft_re;
     d.im = c->dft_im;
+    d.snap = c->dft_snap;
   }
 
   size_t nev = 0;
@@ -583,6 +610,7 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       GD_HIP(c, gd::launch_zstream(t, fields_of(c, c->cur), fields_of(c, 1 - c->cur), m, c->full,
                                    g, s, d, (int)(c->step + r), (int)r, c->stream));
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
+      if (d.ncomp) GD_HIP(c, gd::launch_dft_snapshot(d, t.blt, (int)r, c->stream));
       c->cur = 1 - c->cur;
       r += (size_t)t.blt;
     }

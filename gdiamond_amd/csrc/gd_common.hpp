@@ -31,18 +31,22 @@ struct Material {
 };
 
 // Point sources. The value table[n] enters the component's update at step n.
+// The descriptors live in a small device array (not in the kernel arguments),
+// so kernels index them at run time without holding them in registers.
+struct SourceDesc {
+  int comp, i, j, k;
+  const float* table;  // device, indexed by absolute step
+};
 struct Sources {
   int n;
-  int comp[kMaxSources];
-  int i[kMaxSources], j[kMaxSources], k[kMaxSources];
-  const float* table[kMaxSources];  // device, indexed by absolute step
+  const SourceDesc* d;  // device array of n descriptors
 };
 
 // DFT monitor state (FP64). Weights for run step r (absolute step run_n0 + r)
 // are wc[f*run_len + r] = cos(2 pi f n)*scale and ws likewise with sin.
 struct Dft {
   int ncomp;         // 0 = disabled
-  int comp[6];       // monitored components, slot order
+  unsigned mask;     // monitored components (bit c); slot of c = popcount(mask & ((1<<c)-1))
   int axis;          // 0: i = index, 1: j = index, 2: k = index
   int index;
   int nf;
@@ -53,6 +57,7 @@ struct Dft {
   const double* ws;
   double* re;        // [slot][f][plane]
   double* im;
+  float* snap;       // TILED: plane values per level, [level][component][plane]
 };
 
 struct Grid {
@@ -82,6 +87,22 @@ __device__ __forceinline__ void dft_accumulate(const Dft& d, int slot, long long
   }
 }
 
+// Accumulate the monitored components of one group (E: v = Ex,Ey,Ez; H:
+// v = Hx,Hy,Hz) of one cell. Kept out of line: it runs only on the monitored
+// plane, and inlining its scalar state would crowd the stepping loop.
+template <bool kE>
+__device__ __forceinline__ void dft_accumulate_group(const Dft& d, long long p, float v0,
+                                                               float v1, float v2, int run_step) {
+  const int base = kE ? 0 : 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if ((d.mask >> (base + c)) & 1u) {
+      const int slot = __builtin_popcount(d.mask & ((1u << (base + c)) - 1u));
+      dft_accumulate(d, slot, p, c == 0 ? v0 : (c == 1 ? v1 : v2), run_step);
+    }
+  }
+}
+
 // Source values of the E (J) or H (M) components of one cell at step n.
 // Cells without a source get +0, and x - (+0)*dx == x exactly.
 __device__ __forceinline__ void source_terms(const Sources& s, int base_comp, int i, int j, int k,
@@ -90,10 +111,11 @@ __device__ __forceinline__ void source_terms(const Sources& s, int base_comp, in
   v1 = 0.0f;
   v2 = 0.0f;
   for (int q = 0; q < s.n; ++q) {
-    if (s.i[q] == i && s.j[q] == j && s.k[q] == k) {
-      const int c = s.comp[q] - base_comp;
+    const SourceDesc sd = s.d[q];
+    if (sd.i == i && sd.j == j && sd.k == k) {
+      const int c = sd.comp - base_comp;
       if (c >= 0 && c < 3) {
-        const float v = s.table[q][n];
+        const float v = sd.table[n];
         if (c == 0) v0 = v;
         if (c == 1) v1 = v;
         if (c == 2) v2 = v;
@@ -119,4 +141,8 @@ hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst
                           int run_step0, hipStream_t stream);
 
 hipError_t launch_copy(const float* src, float* dst, size_t n, hipStream_t stream);
+
+// Accumulates the DFT of `nlev` consecutive steps from the per-level plane
+// snapshot the TILED kernel wrote (run steps run_step0 .. run_step0+nlev-1).
+hipError_t launch_dft_snapshot(const Dft& d, int nlev, int run_step0, hipStream_t stream);
 }  // namespace gd

@@ -30,8 +30,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Material m,
   const int k0 = blockIdx.z * kKc;
   const int k1 = min(g.nz, k0 + kKc);
   const bool ij_int = i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2;
-  // DFT slots are ordered by component id, so E components come first.
-  const bool dft_e = d.ncomp > 0 && d.comp[0] < 3;
+  const bool dft_e = (d.mask & 7u) != 0u;
   if (!ij_int && !dft_e) return;
 
   float* __restrict__ Ex = f.c[0];
@@ -83,12 +82,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Material m,
     }
     if (dft_e) {
       const long long p = dft_plane_index(d, g, i, j, k);
-      if (p >= 0) {
-        for (int q = 0; q < d.ncomp; ++q) {
-          const int c = d.comp[q];
-          if (c < 3) dft_accumulate(d, q, p, c == 0 ? ex : (c == 1 ? ey : ez), run_step);
-        }
-      }
+      if (p >= 0) dft_accumulate_group<true>(d, p, ex, ey, ez, run_step);
     }
     hx_km = hx;
     hy_km = hy;
@@ -105,7 +99,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_h_kernel(Fields f, Material m,
   const int k0 = blockIdx.z * kKc;
   const int k1 = min(g.nz, k0 + kKc);
   const bool ij_int = i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2;
-  const bool dft_h = d.ncomp > 0 && (d.comp[d.ncomp - 1] >= 3);
+  const bool dft_h = (d.mask & 56u) != 0u;
   if (!ij_int && !dft_h) return;
 
   const float* __restrict__ Ex = f.c[0];
@@ -159,12 +153,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_h_kernel(Fields f, Material m,
     }
     if (dft_h) {
       const long long p = dft_plane_index(d, g, i, j, k);
-      if (p >= 0) {
-        for (int q = 0; q < d.ncomp; ++q) {
-          const int c = d.comp[q];
-          if (c >= 3) dft_accumulate(d, q, p, c == 3 ? hx : (c == 4 ? hy : hz), run_step);
-        }
-      }
+      if (p >= 0) dft_accumulate_group<false>(d, p, hx, hy, hz, run_step);
     }
     ex = ex_kp;
     ey = ey_kp;

@@ -69,24 +69,47 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Load / store at a per-plane base (uniform, SGPRs) plus a 32-bit byte offset
+// (per lane), so the hardware's scalar-base + vector-offset addressing is used
+// instead of 64-bit per-lane address arithmetic.
+__device__ __forceinline__ float ldp(const char* base, unsigned off) {
+  return *reinterpret_cast<const float*>(base + off);
+}
+__device__ __forceinline__ void stp(char* base, unsigned off, float v) {
+  *reinterpret_cast<float*>(base + off) = v;
+}
+
 // Subtract the point-source terms of step n on plane z from the curl
-// brackets c0..c2 of components base..base+2 (only for columns flagged in
-// `mask`; at most one source per cell and component).
-__device__ __forceinline__ void apply_sources(const Sources& s, unsigned mask, int base, int z,
-                                              int n, float dx, float& c0, float& c1, float& c2) {
-  for (int q = 0; q < s.n; ++q) {
-    if (!((mask >> q) & 1u) || s.k[q] != z) continue;
-    const int c = s.comp[q] - base;
-    if (c < 0 || c > 2) continue;
-    const float v = s.table[q][n];
-    if (c == 0) c0 = c0 - v * dx;
-    if (c == 1) c1 = c1 - v * dx;
-    if (c == 2) c2 = c2 - v * dx;
+// brackets c of components base..base+2, for the sources flagged in `mask`
+// (sources in this column; at most one per cell and component). Out of line:
+// only the few lanes holding a source column ever call it.
+__device__ __forceinline__ float3 apply_sources(const Sources s, unsigned mask, int base,
+                                                          int z, int n, float dx, float3 c) {
+  while (mask) {
+    const int q = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const SourceDesc sd = s.d[q];
+    if (sd.k != z) continue;
+    const int k = sd.comp - base;
+    if (k < 0 || k > 2) continue;
+    const float v = sd.table[n];
+    if (k == 0) c.x = c.x - v * dx;
+    if (k == 1) c.y = c.y - v * dx;
+    if (k == 2) c.z = c.z - v * dx;
   }
+  return c;
+}
+
+// Occupancy target per SIMD: the register state is ~6 values per row per time
+// level, so 2-row threads fit 4 waves per SIMD (<= 128 VGPRs) and 4-row
+// threads 2 waves.
+template <int R>
+constexpr int zstream_waves_per_simd() {
+  return R <= 2 ? 4 : 2;
 }
 
 template <int B, int R, int W, bool kFull>
-__global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
+__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(ZArgs a) {
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
   // lds_h[level-1][parity of p][wave][Hz,Hx][lane]: last row of H of a level,
@@ -105,7 +128,7 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
   const int tz = tyz / a.nty;
 
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-  const size_t sxy = (size_t)a.g.sxy;
+  const size_t plane_bytes = (size_t)a.g.sxy * sizeof(float);
   const float dx = a.g.dx;
 
   const int gx = tx * OX - B + lane;
@@ -114,25 +137,52 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
   const bool ownx = lane >= B && lane < B + OX && gx < nx;
   const unsigned cgx = (unsigned)clampi(gx, 0, nx - 1);
   bool upd_xy[R], own[R];
-  unsigned col[R];     // in-plane offset, clamped into the grid
-  unsigned srow[R];    // point sources in this column (bit q = source q)
+  unsigned off[R];   // in-plane byte offset, clamped into the grid
+  unsigned srow[R];  // point sources in this column (bit q = source q)
+  bool any_src = false;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int gy = gy0 + r;
     upd_xy[r] = intx && gy >= 1 && gy <= ny - 2;
     const int ly = w * R + r;
     own[r] = ownx && ly >= B && ly < B + OY && gy < ny;
-    col[r] = cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx;
+    off[r] = (cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx) * 4u;
     srow[r] = 0u;
-    for (int q = 0; q < a.s.n; ++q)
-      if (a.s.i[q] == gx && a.s.j[q] == gy) srow[r] |= 1u << q;
+    for (int q = 0; q < a.s.n; ++q) {
+      const SourceDesc sd = a.s.d[q];
+      if (sd.i == gx && sd.j == gy) srow[r] |= 1u << q;
+    }
+    any_src = any_src || srow[r] != 0u;
   }
-  const unsigned colm = cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx;
+  const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * 4u;
+  // wave-uniform: does any lane of this wave hold a source column?
+  const bool wave_src = __builtin_amdgcn_ballot_w64(any_src) != 0;
 
   const int z_lo = tz * a.zc;
   const int z_hi = min(nz, z_lo + a.zc);
   const int p_begin = max(0, z_lo - B);
   const int p_end = z_hi - 1 + B;
+
+  // DFT monitor: a cell (x,y,z) of an owned row is on the monitored plane when
+  // dft_row[r] and (axis != 2 or z == index); its plane index is
+  // dft_base[r] + z*dft_zmul (xy plane: x + y*nx; yz plane: y + z*ny; xz
+  // plane: x + z*nx).
+  const bool dft_on = a.d.ncomp > 0;
+  const int dft_zmul = a.d.axis == 2 ? 0 : (a.d.axis == 0 ? ny : nx);
+  bool own_any = false, dft_any = false;
+  bool dft_row[R];
+  int dft_base[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gy = gy0 + r;
+    own_any = own_any || own[r];
+    const bool on = a.d.axis == 2 || (a.d.axis == 0 ? gx == a.d.index : gy == a.d.index);
+    dft_row[r] = dft_on && own[r] && on;
+    dft_base[r] = a.d.axis == 2 ? gx + gy * nx : (a.d.axis == 0 ? gy : gx);
+    dft_any = dft_any || dft_row[r];
+  }
+  // wave-uniform: does this wave own any cell of the monitored plane?
+  const bool wave_dft = __builtin_amdgcn_ballot_w64(dft_any) != 0;
 
   // Pipeline state (registers):
   //   eq[b] = E of level b on plane p-b   (b = 1..B, slot 0 unused)
@@ -155,53 +205,60 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) cbq[b][r] = 0.0f;
 
-  const bool dft_on = a.d.ncomp > 0;
-
   // Exchange rows start at zero; they are only consumed by cells outside the
   // valid cone before the first real write, but stay deterministic.
   for (int q = threadIdx.x; q < B * 2 * W * 2 * 64; q += 64 * W) {
     (&lds_h[0][0][0][0][0])[q] = 0.0f;
     (&lds_e[0][0][0][0][0])[q] = 0.0f;
   }
+
+  // Level-0 values of the next stream position, loaded one iteration ahead so
+  // their HBM latency overlaps the level updates of the current position.
+  // Planes >= nz are clamped to nz-1: they only feed cells outside the cone.
+  float pe[R][3], ph[R][3], pcb[R], phmz, phmx;
+  auto prefetch = [&](int pp) {
+    const size_t pz = (size_t)min(pp, nz - 1) * plane_bytes;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const char* be = reinterpret_cast<const char*>(a.src.c[c]) + pz;
+      const char* bh = reinterpret_cast<const char*>(a.src.c[3 + c]) + pz;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        pe[r][c] = ldp(be, off[r]);
+        ph[r][c] = ldp(bh, off[r]);
+      }
+    }
+    phmz = ldp(reinterpret_cast<const char*>(a.src.c[5]) + pz, offm);
+    phmx = ldp(reinterpret_cast<const char*>(a.src.c[3]) + pz, offm);
+    if (!kFull) {
+      const char* bc = reinterpret_cast<const char*>(a.m.cb) + pz;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pcb[r] = ldp(bc, off[r]);
+    }
+  };
+  prefetch(p_begin);
   __syncthreads();
 
   for (int p = p_begin; p <= p_end; ++p) {
-    // ---- level 0: plane p of the source copy (clamped; planes >= nz only
-    // feed cells outside the cone)
+    // ---- level 0: plane p (prefetched), then start loading plane p+1
     float hc[R][3], ec[R][3];
-    {
-      const size_t pz = (size_t)min(p, nz - 1) * sxy;
-      const float* __restrict__ s0 = a.src.c[0] + pz;
-      const float* __restrict__ s1 = a.src.c[1] + pz;
-      const float* __restrict__ s2 = a.src.c[2] + pz;
-      const float* __restrict__ s3 = a.src.c[3] + pz;
-      const float* __restrict__ s4 = a.src.c[4] + pz;
-      const float* __restrict__ s5 = a.src.c[5] + pz;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        ec[r][0] = s0[col[r]];
-        ec[r][1] = s1[col[r]];
-        ec[r][2] = s2[col[r]];
-        hc[r][0] = s3[col[r]];
-        hc[r][1] = s4[col[r]];
-        hc[r][2] = s5[col[r]];
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ec[r][c] = pe[r][c];
+        hc[r][c] = ph[r][c];
       }
-      if (!kFull) {
-        const float* __restrict__ cbp = a.m.cb + pz;
+    float hmz = phmz, hmx = phmx;  // H of the previous level on row gy0-1
+    if (!kFull) {
 #pragma unroll
-        for (int b = B - 1; b > 0; --b)
+      for (int b = B - 1; b > 0; --b)
 #pragma unroll
-          for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
+        for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
 #pragma unroll
-        for (int r = 0; r < R; ++r) cbq[0][r] = cbp[col[r]];
-      }
+      for (int r = 0; r < R; ++r) cbq[0][r] = pcb[r];
     }
-    float hmz, hmx;  // H of the previous level on row gy0-1 (j-1 of row 0)
-    {
-      const size_t pz = (size_t)min(p, nz - 1) * sxy;
-      hmz = a.src.c[5][pz + colm];
-      hmx = a.src.c[3][pz + colm];
-    }
+    prefetch(p + 1);
 
 #pragma unroll
     for (int b = 1; b <= B; ++b) {
@@ -214,7 +271,7 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
       // ---- E of step n on plane zE from H(level b-1) on zE (hc) and zE-1 (hq[b-1])
       float en[R][3];
       {
-        const size_t oE = (size_t)clampi(zE, 0, nz - 1) * sxy;
+        const size_t oE = (size_t)clampi(zE, 0, nz - 1) * plane_bytes;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const float hz_xm = from_lane_below(hc[r][2]);
@@ -225,12 +282,23 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
           float cx = (hc[r][2] - hz_ym) - (hc[r][1] - hq[b - 1][r][1]);
           float cy = (hc[r][0] - hq[b - 1][r][0]) - (hc[r][2] - hz_xm);
           float cz = (hc[r][1] - hy_xm) - (hc[r][0] - hx_ym);
-          if (srow[r]) apply_sources(a.s, srow[r], 0, zE, n, dx, cx, cy, cz);
+          if (wave_src && srow[r]) {
+            const float3 t = apply_sources(a.s, srow[r], 0, zE, n, dx, make_float3(cx, cy, cz));
+            cx = t.x;
+            cy = t.y;
+            cz = t.z;
+          }
           if (kFull) {
-            const size_t q = oE + col[r];
-            en[r][0] = upd ? a.m.c[0][q] * ec[r][0] + a.m.c[1][q] * cx : ec[r][0];
-            en[r][1] = upd ? a.m.c[2][q] * ec[r][1] + a.m.c[3][q] * cy : ec[r][1];
-            en[r][2] = upd ? a.m.c[4][q] * ec[r][2] + a.m.c[5][q] * cz : ec[r][2];
+            const unsigned o = off[r];
+            const float ca0 = ldp(reinterpret_cast<const char*>(a.m.c[0]) + oE, o);
+            const float cb0 = ldp(reinterpret_cast<const char*>(a.m.c[1]) + oE, o);
+            const float ca1 = ldp(reinterpret_cast<const char*>(a.m.c[2]) + oE, o);
+            const float cb1 = ldp(reinterpret_cast<const char*>(a.m.c[3]) + oE, o);
+            const float ca2 = ldp(reinterpret_cast<const char*>(a.m.c[4]) + oE, o);
+            const float cb2 = ldp(reinterpret_cast<const char*>(a.m.c[5]) + oE, o);
+            en[r][0] = upd ? ca0 * ec[r][0] + cb0 * cx : ec[r][0];
+            en[r][1] = upd ? ca1 * ec[r][1] + cb1 * cy : ec[r][1];
+            en[r][2] = upd ? ca2 * ec[r][2] + cb2 * cz : ec[r][2];
           } else {
             const float cb = cbq[b - 1][r];
             en[r][0] = upd ? ec[r][0] + cb * cx : ec[r][0];
@@ -248,7 +316,7 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
           ez_last = lds_e[b - 1][zH & 1][w + 1][0][lane];
           ex_last = lds_e[b - 1][zH & 1][w + 1][1][lane];
         }
-        const size_t oH = (size_t)clampi(zH, 0, nz - 1) * sxy;
+        const size_t oH = (size_t)clampi(zH, 0, nz - 1) * plane_bytes;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const float ez_xp = from_lane_above(eq[b][r][2]);
@@ -259,12 +327,23 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
           float cx = (en[r][1] - eq[b][r][1]) - (ez_yp - eq[b][r][2]);
           float cy = (ez_xp - eq[b][r][2]) - (en[r][0] - eq[b][r][0]);
           float cz = (ex_yp - eq[b][r][0]) - (ey_xp - eq[b][r][1]);
-          if (srow[r]) apply_sources(a.s, srow[r], 3, zH, n, dx, cx, cy, cz);
+          if (wave_src && srow[r]) {
+            const float3 t = apply_sources(a.s, srow[r], 3, zH, n, dx, make_float3(cx, cy, cz));
+            cx = t.x;
+            cy = t.y;
+            cz = t.z;
+          }
           if (kFull) {
-            const size_t q = oH + col[r];
-            hn[r][0] = upd ? a.m.c[6][q] * hq[b - 1][r][0] + a.m.c[7][q] * cx : hq[b - 1][r][0];
-            hn[r][1] = upd ? a.m.c[8][q] * hq[b - 1][r][1] + a.m.c[9][q] * cy : hq[b - 1][r][1];
-            hn[r][2] = upd ? a.m.c[10][q] * hq[b - 1][r][2] + a.m.c[11][q] * cz : hq[b - 1][r][2];
+            const unsigned o = off[r];
+            const float da0 = ldp(reinterpret_cast<const char*>(a.m.c[6]) + oH, o);
+            const float db0 = ldp(reinterpret_cast<const char*>(a.m.c[7]) + oH, o);
+            const float da1 = ldp(reinterpret_cast<const char*>(a.m.c[8]) + oH, o);
+            const float db1 = ldp(reinterpret_cast<const char*>(a.m.c[9]) + oH, o);
+            const float da2 = ldp(reinterpret_cast<const char*>(a.m.c[10]) + oH, o);
+            const float db2 = ldp(reinterpret_cast<const char*>(a.m.c[11]) + oH, o);
+            hn[r][0] = upd ? da0 * hq[b - 1][r][0] + db0 * cx : hq[b - 1][r][0];
+            hn[r][1] = upd ? da1 * hq[b - 1][r][1] + db1 * cy : hq[b - 1][r][1];
+            hn[r][2] = upd ? da2 * hq[b - 1][r][2] + db2 * cz : hq[b - 1][r][2];
           } else {
             const float db = a.m.db;
             hn[r][0] = upd ? hq[b - 1][r][0] + db * cx : hq[b - 1][r][0];
@@ -274,23 +353,24 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
         }
       }
 
-      // ---- DFT of the values after step n (owned cells of owned planes)
-      if (dft_on) {
-        const int rs = a.run_step0 + b - 1;
-        const bool okE = zE >= z_lo && zE < z_hi;
-        const bool okH = zH >= z_lo && zH < z_hi;
+      // ---- DFT: owners store the values after step n on the monitored plane;
+      // the FP64 accumulation runs after the launch (dft_snapshot_kernel).
+      if (wave_dft) {
+        const bool okE = zE >= z_lo && zE < z_hi && (a.d.axis != 2 || zE == a.d.index);
+        const bool okH = zH >= z_lo && zH < z_hi && (a.d.axis != 2 || zH == a.d.index);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (!own[r]) continue;
-          const long long pe = okE ? dft_plane_index(a.d, a.g, gx, gy0 + r, zE) : -1;
-          const long long ph = okH ? dft_plane_index(a.d, a.g, gx, gy0 + r, zH) : -1;
-          if (pe < 0 && ph < 0) continue;
-          for (int q = 0; q < a.d.ncomp; ++q) {
-            const int c = a.d.comp[q];
-            if (c < 3 && pe >= 0)
-              dft_accumulate(a.d, q, pe, c == 0 ? en[r][0] : (c == 1 ? en[r][1] : en[r][2]), rs);
-            if (c >= 3 && ph >= 0)
-              dft_accumulate(a.d, q, ph, c == 3 ? hn[r][0] : (c == 4 ? hn[r][1] : hn[r][2]), rs);
+        for (int c = 0; c < 3; ++c) {
+          if (okE && ((a.d.mask >> c) & 1u)) {
+            float* sp = a.d.snap + ((size_t)(b - 1) * 6 + c) * (size_t)a.d.plane;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (dft_row[r]) sp[dft_base[r] + zE * dft_zmul] = en[r][c];
+          }
+          if (okH && ((a.d.mask >> (3 + c)) & 1u)) {
+            float* sp = a.d.snap + ((size_t)(b - 1) * 6 + 3 + c) * (size_t)a.d.plane;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (dft_row[r]) sp[dft_base[r] + zH * dft_zmul] = hn[r][c];
           }
         }
       }
@@ -304,31 +384,31 @@ __global__ __launch_bounds__(64 * W) void zstream_kernel(ZArgs a) {
       }
 
       // ---- final level: owned cells go to the destination copy
-      if (b == B) {
+      if (b == B && own_any) {
         if (zE >= z_lo && zE < z_hi) {
-          const size_t oE = (size_t)zE * sxy;
-          float* __restrict__ d0 = a.dst.c[0] + oE;
-          float* __restrict__ d1 = a.dst.c[1] + oE;
-          float* __restrict__ d2 = a.dst.c[2] + oE;
+          const size_t oE = (size_t)zE * plane_bytes;
+          char* d0 = reinterpret_cast<char*>(a.dst.c[0]) + oE;
+          char* d1 = reinterpret_cast<char*>(a.dst.c[1]) + oE;
+          char* d2 = reinterpret_cast<char*>(a.dst.c[2]) + oE;
 #pragma unroll
           for (int r = 0; r < R; ++r)
             if (own[r]) {
-              d0[col[r]] = en[r][0];
-              d1[col[r]] = en[r][1];
-              d2[col[r]] = en[r][2];
+              stp(d0, off[r], en[r][0]);
+              stp(d1, off[r], en[r][1]);
+              stp(d2, off[r], en[r][2]);
             }
         }
         if (zH >= z_lo && zH < z_hi) {
-          const size_t oH = (size_t)zH * sxy;
-          float* __restrict__ d3 = a.dst.c[3] + oH;
-          float* __restrict__ d4 = a.dst.c[4] + oH;
-          float* __restrict__ d5 = a.dst.c[5] + oH;
+          const size_t oH = (size_t)zH * plane_bytes;
+          char* d3 = reinterpret_cast<char*>(a.dst.c[3]) + oH;
+          char* d4 = reinterpret_cast<char*>(a.dst.c[4]) + oH;
+          char* d5 = reinterpret_cast<char*>(a.dst.c[5]) + oH;
 #pragma unroll
           for (int r = 0; r < R; ++r)
             if (own[r]) {
-              d3[col[r]] = hn[r][0];
-              d4[col[r]] = hn[r][1];
-              d5[col[r]] = hn[r][2];
+              stp(d3, off[r], hn[r][0]);
+              stp(d4, off[r], hn[r][1]);
+              stp(d5, off[r], hn[r][2]);
             }
         }
       }
@@ -376,7 +456,33 @@ hipError_t launch_rw(int blt, const ZArgs& a, bool full, hipStream_t stream) {
   }
 }
 
+__global__ __launch_bounds__(256) void dft_snapshot_kernel(Dft d, int nlev, int run_step0) {
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= d.plane) return;
+  for (int c = 0; c < 6; ++c) {
+    if (!((d.mask >> c) & 1u)) continue;
+    const int slot = __builtin_popcount(d.mask & ((1u << c) - 1u));
+    for (int f = 0; f < d.nf; ++f) {
+      const long long o = ((long long)slot * d.nf + f) * d.plane + p;
+      double re = d.re[o], im = d.im[o];
+      for (int b = 0; b < nlev; ++b) {
+        const double v = (double)d.snap[((long long)b * 6 + c) * d.plane + p];
+        re = re + v * d.wc[(long long)f * d.run_len + run_step0 + b];
+        im = im - v * d.ws[(long long)f * d.run_len + run_step0 + b];
+      }
+      d.re[o] = re;
+      d.im[o] = im;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_dft_snapshot(const Dft& d, int nlev, int run_step0, hipStream_t stream) {
+  const unsigned blocks = (unsigned)((d.plane + 255) / 256);
+  hipLaunchKernelGGL(dft_snapshot_kernel, dim3(blocks), dim3(256), 0, stream, d, nlev, run_step0);
+  return hipGetLastError();
+}
 
 bool ztiling_supported(const ZTiling& t, bool /*full*/) {
   if (t.blt < 1 || t.blt > 4 || t.zchunk < 1) return false;
@@ -390,6 +496,8 @@ hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst
                           const Material& m, bool full, const Grid& g, const Sources& s,
                           const Dft& d, int n0, int run_step0, hipStream_t stream) {
   if (!ztiling_supported(t, full)) return hipErrorInvalidValue;
+  // in-plane byte offsets are 32-bit
+  if ((unsigned long long)g.sxy * 4ull >= (1ull << 32)) return hipErrorInvalidValue;
   ZArgs a;
   a.src = src;
   a.dst = dst;

@@ -23,6 +23,7 @@ TILINGS = [
     dict(blt=4, rows=4, waves=4, zchunk=9),
     dict(blt=3, rows=2, waves=4, zchunk=3),
     dict(blt=4, rows=2, waves=8, zchunk=16),
+    dict(blt=4, rows=4, waves=8, zchunk=11),
 ]
 
 
@@ -98,7 +99,7 @@ def test_reference_default_problem_golden(gpu, name, variant):
         got, _ = run_variant(gpu, (nx, ny, nz), setup, T, gpu.PLAIN)
     else:
         t = (dict(blt=3, rows=2, waves=4, zchunk=5) if variant == "tiled3"
-             else dict(blt=4, rows=4, waves=4, zchunk=6))
+             else dict(blt=4, rows=4, waves=8, zchunk=6))
         got, _ = run_variant(gpu, (nx, ny, nz), setup, T, gpu.TILED, t)
     assert_fields_equal(got, [g[c] for c in gpu.COMPONENTS], f"{name}/{variant}")
 
