@@ -41,20 +41,21 @@ def load_metric():
         return "Gcell-updates/s at 512^3 + % HBM roofline"
 
 
-def parse_traffic(kernel_key):
-    """HBM bytes per launch of the stepping kernel, from the committed rocprofv3
-    PMC summary (profiles/<kernel>_traffic.txt, line 'bytes_per_launch: X')."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_key}_traffic.txt")
-    if not os.path.exists(path):
+def parse_traffic(path, tiling, n):
+    """HBM bytes per launch of the stepping kernel from a committed rocprofv3
+    PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration). Used only when it was measured for this tiling and size."""
+    if not os.path.exists(path) or n != 512:
         return None
+    want = "tiling: " + " ".join(f"{k}={tiling[k]}" for k in ("blt", "rows", "waves", "zchunk"))
+    found, val = False, None
     with open(path) as f:
         for ln in f:
+            if ln.strip() == want:
+                found = True
             if ln.startswith("bytes_per_launch:"):
-                try:
-                    return float(ln.split(":", 1)[1])
-                except ValueError:
-                    return None
-    return None
+                val = float(ln.split(":", 1)[1])
+    return val if found else None
 
 
 def build_problem(gd, n, timesteps, seed=42):
@@ -165,14 +166,15 @@ def main():
     mean_launch_ms = sum(kernel_ms) / max(1, nlaunch * args.steps)
     # algorithmic bytes of all launches of one run / their summed duration
     achieved = B_ALG_FP32 * cells * T / (np.mean(kernel_ms) * 1e-3) / 1e9
-    kernel_key = "zstream" if variant == gd.TILED else "plain"
-    traffic = parse_traffic(kernel_key) if variant == gd.TILED else None
+    traffic_file = os.path.join(ROOT, "profiles", "zstream_512cube_traffic.txt")
+    traffic = parse_traffic(traffic_file, tiling, n) if variant == gd.TILED else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "zstream_kernel" if variant == gd.TILED else "plain_e/h_kernel",
                 "bytes_alg_per_cell_update": B_ALG_FP32,
                 "mean_launch_ms": round(mean_launch_ms, 4),
-                "launches_per_run": nlaunch}
+                "launches_per_run": nlaunch,
+                "traffic_source": "profiles/zstream_512cube_traffic.txt" if traffic else None}
 
     out = {
         "metric": load_metric(),
