@@ -28,7 +28,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-B_ALG_FP32 = 52  # bytes per cell-update: 6 fields x 4 B x (read + write) + one FP32 coefficient
+# algorithmic bytes per cell-update: 6 fields x (read + write) + the material
+# (a 1-byte material id for the indexed form, one Cb in the storage format for
+# the dense form)
+FIELD_BYTES = {"f32": 4, "f16": 2}
+
+
+def bytes_alg(storage, material):
+    return 6 * FIELD_BYTES[storage] * 2 + (1 if material == "indexed" else FIELD_BYTES[storage])
+F16_SCALE_EXP = -8  # binary16 range of the stored fields: ~2.4e-7 .. 256
 HBM_PEAK_GBS = 8000.0
 METRIC = None
 
@@ -41,21 +49,23 @@ def load_metric():
         return "Gcell-updates/s at 512^3 + % HBM roofline"
 
 
-def parse_traffic(path, tiling, n):
+def parse_traffic(path, tiling, n, storage, material):
     """HBM bytes per launch of the stepping kernel from a committed rocprofv3
     PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
-    calibration). Used only when it was measured for this tiling and size."""
+    calibration). Used only when it was measured for this tiling, size,
+    storage and material."""
     if not os.path.exists(path) or n != 512:
         return None
-    want = "tiling: " + " ".join(f"{k}={tiling[k]}" for k in ("blt", "rows", "waves", "zchunk"))
-    found, val = False, None
+    want = {"tiling: " + " ".join(f"{k}={tiling[k]}" for k in ("blt", "rows", "waves", "zchunk")),
+            f"storage: {storage}", f"material: {material}"}
+    seen, val = set(), None
     with open(path) as f:
         for ln in f:
-            if ln.strip() == want:
-                found = True
+            if ln.strip() in want:
+                seen.add(ln.strip())
             if ln.startswith("bytes_per_launch:"):
                 val = float(ln.split(":", 1)[1])
-    return val if found else None
+    return val if seen == want else None
 
 
 def build_problem(gd, n, timesteps, seed=42):
@@ -91,6 +101,10 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--timesteps", type=int, default=1000)
     ap.add_argument("--variant", choices=["tiled", "plain"], default="tiled")
+    ap.add_argument("--storage", choices=["f32", "f16"], default="f32",
+                    help="field storage: FP32, or binary16 with scale 2^%d" % F16_SCALE_EXP)
+    ap.add_argument("--material", choices=["auto", "dense", "indexed"], default="auto",
+                    help="device form of the isotropic Cb (auto: indexed for <= 256 values)")
     ap.add_argument("--blt", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
@@ -120,7 +134,13 @@ def main():
     t_setup = time.perf_counter()
     dt, eps, src = build_problem(gd, n, T)
     ctx = gd.Context(n, n, n, dx=1.0)
+    if args.storage == "f16":
+        ctx.set_storage(gd.F16, [F16_SCALE_EXP] * 6)
+    ctx.set_material_format({"auto": gd.MAT_AUTO, "dense": gd.MAT_DENSE,
+                             "indexed": gd.MAT_INDEXED}[args.material])
     ctx.set_isotropic_eps(eps, dt)
+    mat_fmt, ntable = ctx.get_material_format()
+    material = "indexed" if mat_fmt == gd.MAT_INDEXED else "dense"
     c = n // 2
     ctx.add_source(gd.EZ, c, c, c, src)
     f_cps = np.array([0.03, 0.05, 0.07])
@@ -165,16 +185,19 @@ def main():
     ms_per_step = total / args.steps * 1e3
     mean_launch_ms = sum(kernel_ms) / max(1, nlaunch * args.steps)
     # algorithmic bytes of all launches of one run / their summed duration
-    achieved = B_ALG_FP32 * cells * T / (np.mean(kernel_ms) * 1e-3) / 1e9
-    traffic_file = os.path.join(ROOT, "profiles", "zstream_512cube_traffic.txt")
-    traffic = parse_traffic(traffic_file, tiling, n) if variant == gd.TILED else None
+    b_alg = bytes_alg(args.storage, material)
+    achieved = b_alg * cells * T / (np.mean(kernel_ms) * 1e-3) / 1e9
+    traffic_name = f"zstream_512cube_{args.storage}_{material}_traffic.txt"
+    traffic_file = os.path.join(ROOT, "profiles", traffic_name)
+    traffic = (parse_traffic(traffic_file, tiling, n, args.storage, material)
+               if variant == gd.TILED else None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "zstream_kernel" if variant == gd.TILED else "plain_e/h_kernel",
-                "bytes_alg_per_cell_update": B_ALG_FP32,
+                "bytes_alg_per_cell_update": b_alg,
                 "mean_launch_ms": round(mean_launch_ms, 4),
                 "launches_per_run": nlaunch,
-                "traffic_source": "profiles/zstream_512cube_traffic.txt" if traffic else None}
+                "traffic_source": "profiles/" + traffic_name if traffic else None}
 
     out = {
         "metric": load_metric(),
@@ -187,16 +210,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.storage,
         "data": "synthetic",
         "config": {"workload": f"fdtd3d-yee-spheres-{n}^3-{T}steps-dft3",
                    "grid": [n, n, n], "timesteps": T, "medium": "200 spheres seed 42, eps 1-12",
                    "source": "Gaussian Jz at centre", "dft": "Ex,Ez z=256 f=0.03/0.05/0.07 fp64",
-                   "variant": args.variant, "tiling": tiling, "parallelism": "single-gpu",
+                   "variant": args.variant, "storage": args.storage,
+                   "material": f"{material} ({ntable} values)" if ntable else material,
+                   "tiling": tiling, "parallelism": "single-gpu",
                    "dft_finite": finite},
         "roofline": roofline,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.storage == "f32":
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(gd, n, eps, dt, src, args.cpu_steps, threads)
         out["cpu_baseline"] = {"value": round(cpu, 4), "unit": "Gcell-updates/s",

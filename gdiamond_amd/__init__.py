@@ -24,6 +24,8 @@ COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 COEFF_NAMES = ("Cax", "Cbx", "Cay", "Cby", "Caz", "Cbz",
                "Dax", "Dbx", "Day", "Dby", "Daz", "Dbz")
 PLAIN, TILED = 0, 1
+F32, F64, F16 = 0, 1, 2  # storage formats
+MAT_AUTO, MAT_DENSE, MAT_INDEXED, MAT_FULL = 0, 1, 2, 3  # material formats
 
 GD_OK = 0
 ERRORS = {-1: "GD_ERR_ARG", -2: "GD_ERR_STATE", -3: "GD_ERR_HIP", -4: "GD_ERR_NOMEM",
@@ -55,9 +57,17 @@ _SIGS = {
                             C.c_float]),
     "gd_destroy": (C.c_int, [C.c_void_p]),
     "gd_last_error": (C.c_char_p, [C.c_void_p]),
+    "gd_set_storage": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "gd_get_storage": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gd_get_fields_f64": (C.c_int, [C.c_void_p, _DP * 6]),
+    "gd_get_field_f64": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "gd_set_coefficients": (C.c_int, [C.c_void_p, _FP * 12]),
     "gd_set_isotropic": (C.c_int, [C.c_void_p, _FP, C.c_float]),
     "gd_set_isotropic_eps": (C.c_int, [C.c_void_p, _FP, C.c_float]),
+    "gd_set_isotropic_indexed": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), _FP, C.c_int,
+                                           C.c_float]),
+    "gd_set_material_format": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_get_material_format": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gd_add_source": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, _FP,
                                 C.c_size_t]),
     "gd_clear_sources": (C.c_int, [C.c_void_p]),
@@ -220,6 +230,23 @@ class Context:
         nx, ny, nz = self.shape
         return (nz, ny, nx)
 
+    # -- storage
+    def set_storage(self, storage, scale_exp=None):
+        se = None
+        if scale_exp is not None:
+            se = (C.c_int * 6)(*[int(v) for v in scale_exp])
+        self._ok(lib().gd_set_storage(self._h, int(storage), se))
+
+    def get_storage(self):
+        st, se = C.c_int(), (C.c_int * 6)()
+        self._ok(lib().gd_get_storage(self._h, C.byref(st), se))
+        return st.value, list(se)
+
+    def get_fields_f64(self):
+        out = [np.empty(self.zyx, np.float64) for _ in range(6)]
+        self._ok(lib().gd_get_fields_f64(self._h, (_DP * 6)(*[_dp(a) for a in out])))
+        return out
+
     # -- material
     def set_coefficients(self, coeffs):
         arrs = [_f32(c, self.N) for c in coeffs]
@@ -234,6 +261,22 @@ class Context:
     def set_isotropic_eps(self, eps_r, dt):
         a = _f32(eps_r, self.N)
         self._ok(lib().gd_set_isotropic_eps(self._h, _fp(a), float(dt)))
+
+    def set_isotropic_indexed(self, ids, table, db):
+        i = np.ascontiguousarray(ids, np.uint8).ravel()
+        if i.size != self.N:
+            raise ValueError(f"expected {self.N} material ids, got {i.size}")
+        t = np.ascontiguousarray(table, np.float32).ravel()
+        self._ok(lib().gd_set_isotropic_indexed(self._h, i.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                _fp(t), t.size, float(db)))
+
+    def set_material_format(self, fmt):
+        self._ok(lib().gd_set_material_format(self._h, int(fmt)))
+
+    def get_material_format(self):
+        f, n = C.c_int(), C.c_int()
+        self._ok(lib().gd_get_material_format(self._h, C.byref(f), C.byref(n)))
+        return f.value, n.value
 
     # -- sources
     def add_source(self, comp, i, j, k, table):

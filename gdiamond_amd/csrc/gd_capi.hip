@@ -19,19 +19,28 @@
 
 #include "../../include/gdiamond/gd_capi.h"
 #include "gd_common.hpp"
+#include "gd_half.hpp"
 
 struct gd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t nx = 0, ny = 0, nz = 0, N = 0;
   float dx = 1.0f;
-  float* f[2][6] = {{nullptr}};
+  void* f[2][6] = {{nullptr}};  // field copies, element type per storage
   int cur = 0;
   size_t step = 0;
+  int storage = GD_STORAGE_F32;
+  int scale_exp[6] = {0, 0, 0, 0, 0, 0};
+  size_t esz = sizeof(float);  // bytes per stored field value
 
-  bool have_mat = false, full = false;
-  float* coeff[12] = {nullptr};
-  float* cb = nullptr;
+  bool have_mat = false;
+  int mat_req = GD_MATERIAL_AUTO;  // requested isotropic format
+  gd::MatKind mat_kind = gd::kMatIso;
+  void* coeff[12] = {nullptr};  // full material, compute type (float / double)
+  void* cb = nullptr;           // dense isotropic coefficient, storage type
+  uint8_t* mat_id = nullptr;    // indexed isotropic: id per cell
+  float* mat_table = nullptr;   // indexed isotropic: Cb per id (GD_MAX_MATERIALS floats)
+  int ntable = 0;
   float db = 0.0f;
 
   struct Src {
@@ -98,11 +107,51 @@ __attribute__((noinline)) double weight_sin(double a) { return std::sin(a); }
 int alloc_fields(gd_ctx* c, int copy) {
   for (int a = 0; a < 6; ++a) {
     if (c->f[copy][a]) continue;
-    hipError_t e = hipMalloc(&c->f[copy][a], c->N * sizeof(float));
+    hipError_t e = hipMalloc(&c->f[copy][a], c->N * c->esz);
     if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(fields)");
-    GD_HIP(c, hipMemsetAsync(c->f[copy][a], 0, c->N * sizeof(float), c->stream));
+    GD_HIP(c, hipMemsetAsync(c->f[copy][a], 0, c->N * c->esz, c->stream));
   }
   return GD_OK;
+}
+
+float scale_of(const gd_ctx* c, int comp) { return std::ldexp(1.0f, c->scale_exp[comp]); }
+
+// Host values -> storage bytes of one component (fp32 host values).
+void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<unsigned char>& out) {
+  out.resize(c->N * c->esz);
+  if (c->storage == GD_STORAGE_F32) {
+    std::memcpy(out.data(), in, c->N * sizeof(float));
+  } else if (c->storage == GD_STORAGE_F64) {
+    double* o = reinterpret_cast<double*>(out.data());
+    for (size_t q = 0; q < c->N; ++q) o[q] = (double)in[q];
+  } else {
+    const float inv = std::ldexp(1.0f, -c->scale_exp[comp]);
+    uint16_t* o = reinterpret_cast<uint16_t*>(out.data());
+    for (size_t q = 0; q < c->N; ++q) o[q] = gd::f32_to_f16(in[q] * inv);
+  }
+}
+
+// Storage bytes -> double values (exact for every format).
+void decode_to_f64(const gd_ctx* c, int comp, const unsigned char* in, double* out, size_t n) {
+  if (c->storage == GD_STORAGE_F32) {
+    const float* v = reinterpret_cast<const float*>(in);
+    for (size_t q = 0; q < n; ++q) out[q] = v[q];
+  } else if (c->storage == GD_STORAGE_F64) {
+    std::memcpy(out, in, n * sizeof(double));
+  } else {
+    const float s = scale_of(c, comp);
+    const uint16_t* v = reinterpret_cast<const uint16_t*>(in);
+    for (size_t q = 0; q < n; ++q) out[q] = (double)(gd::f16_to_f32(v[q]) * s);
+  }
+}
+
+gd::Scales scales_of(const gd_ctx* c) {
+  gd::Scales sc;
+  for (int a = 0; a < 6; ++a) {
+    sc.s[a] = std::ldexp(1.0f, c->scale_exp[a]);
+    sc.inv[a] = std::ldexp(1.0f, -c->scale_exp[a]);
+  }
+  return sc;
 }
 
 void free_material(gd_ctx* c) {
@@ -112,7 +161,64 @@ void free_material(gd_ctx* c) {
   }
   if (c->cb) (void)hipFree(c->cb);
   c->cb = nullptr;
+  if (c->mat_id) (void)hipFree(c->mat_id);
+  c->mat_id = nullptr;
+  if (c->mat_table) (void)hipFree(c->mat_table);
+  c->mat_table = nullptr;
+  c->ntable = 0;
   c->have_mat = false;
+}
+
+// Material ids of an isotropic Cb array when it holds at most
+// GD_MAX_MATERIALS distinct bit patterns (ids in order of first appearance).
+// Returns false when there are more.
+bool index_materials(const float* cb, size_t n, std::vector<uint8_t>& id,
+                     std::vector<float>& table) {
+  constexpr unsigned kSlots = 1024;  // open addressing, load <= 1/4
+  uint32_t key[kSlots];
+  int val[kSlots];
+  for (unsigned q = 0; q < kSlots; ++q) val[q] = -1;
+  id.resize(n);
+  table.clear();
+  uint32_t last_bits = 0;
+  int last_id = -1;
+  for (size_t q = 0; q < n; ++q) {
+    uint32_t bits;
+    std::memcpy(&bits, &cb[q], 4);
+    if (last_id < 0 || bits != last_bits) {
+      unsigned h = (bits * 2654435761u) >> 22;
+      while (val[h] >= 0 && key[h] != bits) h = (h + 1) & (kSlots - 1);
+      if (val[h] < 0) {
+        if ((int)table.size() == GD_MAX_MATERIALS) return false;
+        key[h] = bits;
+        val[h] = (int)table.size();
+        table.push_back(cb[q]);
+      }
+      last_bits = bits;
+      last_id = val[h];
+    }
+    id[q] = (uint8_t)last_id;
+  }
+  return true;
+}
+
+int upload_indexed(gd_ctx* c, const uint8_t* id, const float* table, int ntable, float db) {
+  free_material(c);
+  hipError_t e = hipMalloc(&c->mat_id, c->N);
+  if (e == hipSuccess) e = hipMalloc(&c->mat_table, GD_MAX_MATERIALS * sizeof(float));
+  if (e != hipSuccess) {
+    free_material(c);
+    return c->hip_fail(e, "hipMalloc(material ids)");
+  }
+  float padded[GD_MAX_MATERIALS] = {0.0f};
+  std::memcpy(padded, table, (size_t)ntable * sizeof(float));
+  GD_HIP(c, hipMemcpy(c->mat_id, id, c->N, hipMemcpyHostToDevice));
+  GD_HIP(c, hipMemcpy(c->mat_table, padded, sizeof(padded), hipMemcpyHostToDevice));
+  c->ntable = ntable;
+  c->db = db;
+  c->mat_kind = gd::kMatIdx;
+  c->have_mat = true;
+  return GD_OK;
 }
 
 gd::Fields fields_of(gd_ctx* c, int copy) {
@@ -230,6 +336,45 @@ const char* gd_last_error(const gd_ctx* c) {
   return c->err.c_str();
 }
 
+// ---- storage ---------------------------------------------------------------
+
+int gd_set_storage(gd_ctx* c, int storage, const int* scale_exp) {
+  if (!c) return GD_ERR_ARG;
+  if (storage != GD_STORAGE_F32 && storage != GD_STORAGE_F64 && storage != GD_STORAGE_F16)
+    return c->fail(GD_ERR_ARG, "unknown storage format");
+  if (c->have_mat) return c->fail(GD_ERR_STATE, "set the storage format before the material");
+  int se[6] = {0, 0, 0, 0, 0, 0};
+  if (scale_exp) {
+    for (int a = 0; a < 6; ++a) {
+      if (scale_exp[a] < -100 || scale_exp[a] > 100)
+        return c->fail(GD_ERR_ARG, "scale exponents must be in -100..100");
+      se[a] = scale_exp[a];
+    }
+  }
+  GD_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 2; ++k)
+    for (int a = 0; a < 6; ++a) {
+      if (c->f[k][a]) (void)hipFree(c->f[k][a]);
+      c->f[k][a] = nullptr;
+    }
+  c->storage = storage;
+  std::memcpy(c->scale_exp, se, sizeof(se));
+  c->esz = storage == GD_STORAGE_F64 ? 8 : (storage == GD_STORAGE_F16 ? 2 : 4);
+  c->cur = 0;
+  c->step = 0;
+  int rc = alloc_fields(c, 0);
+  if (rc != GD_OK) return rc;
+  GD_HIP(c, hipStreamSynchronize(c->stream));
+  return GD_OK;
+}
+
+int gd_get_storage(const gd_ctx* c, int* storage, int* scale_exp) {
+  if (!c || !storage) return GD_ERR_ARG;
+  *storage = c->storage;
+  if (scale_exp) std::memcpy(scale_exp, c->scale_exp, sizeof(c->scale_exp));
+  return GD_OK;
+}
+
 // ---- material -------------------------------------------------------------
 
 int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
@@ -237,31 +382,109 @@ int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
   if (!coeff) return c->fail(GD_ERR_ARG, "null coefficient table");
   for (int a = 0; a < 12; ++a)
     if (!coeff[a]) return c->fail(GD_ERR_ARG, "null coefficient array");
+  if (c->storage == GD_STORAGE_F16)
+    return c->fail(GD_ERR_UNSUPPORTED, "16-bit storage supports the isotropic material only");
   free_material(c);
+  const size_t es = c->storage == GD_STORAGE_F64 ? sizeof(double) : sizeof(float);
+  std::vector<double> tmp;
   for (int a = 0; a < 12; ++a) {
-    hipError_t e = hipMalloc(&c->coeff[a], c->N * sizeof(float));
+    hipError_t e = hipMalloc(&c->coeff[a], c->N * es);
     if (e != hipSuccess) {
       free_material(c);
       return c->hip_fail(e, "hipMalloc(coefficients)");
     }
-    GD_HIP(c, hipMemcpy(c->coeff[a], coeff[a], c->N * sizeof(float), hipMemcpyHostToDevice));
+    if (c->storage == GD_STORAGE_F64) {
+      tmp.assign(coeff[a], coeff[a] + c->N);
+      GD_HIP(c, hipMemcpy(c->coeff[a], tmp.data(), c->N * es, hipMemcpyHostToDevice));
+    } else {
+      GD_HIP(c, hipMemcpy(c->coeff[a], coeff[a], c->N * es, hipMemcpyHostToDevice));
+    }
   }
-  c->full = true;
+  c->mat_kind = gd::kMatFull;
   c->have_mat = true;
+  return GD_OK;
+}
+
+int gd_set_material_format(gd_ctx* c, int format) {
+  if (!c) return GD_ERR_ARG;
+  if (format != GD_MATERIAL_AUTO && format != GD_MATERIAL_DENSE && format != GD_MATERIAL_INDEXED)
+    return c->fail(GD_ERR_ARG, "unknown material format");
+  if (c->have_mat) return c->fail(GD_ERR_STATE, "set the material format before the material");
+  c->mat_req = format;
+  return GD_OK;
+}
+
+int gd_get_material_format(const gd_ctx* c, int* format, int* ntable) {
+  if (!c || !format) return GD_ERR_ARG;
+  if (!c->have_mat)
+    *format = -1;
+  else
+    *format = c->mat_kind == gd::kMatFull ? GD_MATERIAL_FULL
+              : c->mat_kind == gd::kMatIdx ? GD_MATERIAL_INDEXED
+                                           : GD_MATERIAL_DENSE;
+  if (ntable) *ntable = c->have_mat && c->mat_kind == gd::kMatIdx ? c->ntable : 0;
   return GD_OK;
 }
 
 int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (!c) return GD_ERR_ARG;
   if (!cb) return c->fail(GD_ERR_ARG, "null cb array");
+  if (c->mat_req != GD_MATERIAL_DENSE) {
+    if (c->storage == GD_STORAGE_F64) {
+      if (c->mat_req == GD_MATERIAL_INDEXED)
+        return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or FP16 storage");
+    } else {
+      std::vector<uint8_t> id;
+      std::vector<float> table;
+      if (index_materials(cb, c->N, id, table))
+        return upload_indexed(c, id.data(), table.data(), (int)table.size(), db);
+      if (c->mat_req == GD_MATERIAL_INDEXED)
+        return c->fail(GD_ERR_UNSUPPORTED, "more than 256 distinct Cb values for the indexed "
+                                           "material");
+    }
+  }
   free_material(c);
-  hipError_t e = hipMalloc(&c->cb, c->N * sizeof(float));
+  hipError_t e = hipMalloc(&c->cb, c->N * c->esz);
   if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(cb)");
-  GD_HIP(c, hipMemcpy(c->cb, cb, c->N * sizeof(float), hipMemcpyHostToDevice));
+  // the coefficient is stored in the field storage format (binary16 without
+  // scaling for 16-bit storage)
+  std::vector<unsigned char> buf(c->N * c->esz);
+  if (c->storage == GD_STORAGE_F32) {
+    std::memcpy(buf.data(), cb, c->N * sizeof(float));
+  } else if (c->storage == GD_STORAGE_F64) {
+    double* o = reinterpret_cast<double*>(buf.data());
+    for (size_t q = 0; q < c->N; ++q) o[q] = (double)cb[q];
+  } else {
+    uint16_t* o = reinterpret_cast<uint16_t*>(buf.data());
+    for (size_t q = 0; q < c->N; ++q) o[q] = gd::f32_to_f16(cb[q]);
+  }
+  GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
   c->db = db;
-  c->full = false;
+  c->mat_kind = gd::kMatIso;
   c->have_mat = true;
   return GD_OK;
+}
+
+int gd_set_isotropic_indexed(gd_ctx* c, const uint8_t* id, const float* table, int ntable,
+                             float db) {
+  if (!c) return GD_ERR_ARG;
+  if (!id || !table) return c->fail(GD_ERR_ARG, "null id or table");
+  if (ntable < 1 || ntable > GD_MAX_MATERIALS)
+    return c->fail(GD_ERR_ARG, "ntable must be in 1..256");
+  for (size_t q = 0; q < c->N; ++q)
+    if ((int)id[q] >= ntable) return c->fail(GD_ERR_ARG, "material id out of range");
+  if (c->storage == GD_STORAGE_F64 || c->mat_req == GD_MATERIAL_DENSE) {
+    if (c->storage == GD_STORAGE_F64 && c->mat_req == GD_MATERIAL_INDEXED)
+      return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or FP16 storage");
+    std::vector<float> cb(c->N);
+    for (size_t q = 0; q < c->N; ++q) cb[q] = table[id[q]];
+    const int req = c->mat_req;
+    c->mat_req = GD_MATERIAL_DENSE;
+    const int rc = gd_set_isotropic(c, cb.data(), db);
+    c->mat_req = req;
+    return rc;
+  }
+  return upload_indexed(c, id, table, ntable, db);
 }
 
 int gd_set_isotropic_eps(gd_ctx* c, const float* eps_r, float dt) {
@@ -318,8 +541,21 @@ int gd_set_fields(gd_ctx* c, const float* const fields[6]) {
   for (int a = 0; a < 6; ++a)
     if (!fields[a]) return c->fail(GD_ERR_ARG, "null field array");
   GD_HIP(c, hipStreamSynchronize(c->stream));
-  for (int a = 0; a < 6; ++a)
-    GD_HIP(c, hipMemcpy(c->f[c->cur][a], fields[a], c->N * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<unsigned char> buf;
+  for (int a = 0; a < 6; ++a) {
+    encode_component(c, a, fields[a], buf);
+    GD_HIP(c, hipMemcpy(c->f[c->cur][a], buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
+  }
+  return GD_OK;
+}
+
+int gd_get_field_f64(gd_ctx* c, int comp, double* out) {
+  if (!c) return GD_ERR_ARG;
+  if (comp < 0 || comp > 5 || !out) return c->fail(GD_ERR_ARG, "bad component or null output");
+  GD_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<unsigned char> buf(c->N * c->esz);
+  GD_HIP(c, hipMemcpy(buf.data(), c->f[c->cur][comp], c->N * c->esz, hipMemcpyDeviceToHost));
+  decode_to_f64(c, comp, buf.data(), out, c->N);
   return GD_OK;
 }
 
@@ -337,7 +573,24 @@ int gd_get_field(gd_ctx* c, int comp, float* out) {
   if (!c) return GD_ERR_ARG;
   if (comp < 0 || comp > 5 || !out) return c->fail(GD_ERR_ARG, "bad component or null output");
   GD_HIP(c, hipStreamSynchronize(c->stream));
-  GD_HIP(c, hipMemcpy(out, c->f[c->cur][comp], c->N * sizeof(float), hipMemcpyDeviceToHost));
+  if (c->storage == GD_STORAGE_F32) {
+    GD_HIP(c, hipMemcpy(out, c->f[c->cur][comp], c->N * sizeof(float), hipMemcpyDeviceToHost));
+    return GD_OK;
+  }
+  std::vector<double> v(c->N);
+  int rc = gd_get_field_f64(c, comp, v.data());
+  if (rc != GD_OK) return rc;
+  for (size_t q = 0; q < c->N; ++q) out[q] = (float)v[q];
+  return GD_OK;
+}
+
+int gd_get_fields_f64(gd_ctx* c, double* const fields[6]) {
+  if (!c) return GD_ERR_ARG;
+  if (!fields) return c->fail(GD_ERR_ARG, "null field table");
+  for (int a = 0; a < 6; ++a) {
+    int rc = gd_get_field_f64(c, a, fields[a]);
+    if (rc != GD_OK) return rc;
+  }
   return GD_OK;
 }
 
@@ -346,15 +599,20 @@ int gd_get_plane_xy(gd_ctx* c, int comp, size_t k, float* out) {
   if (comp < 0 || comp > 5 || !out || k >= c->nz)
     return c->fail(GD_ERR_ARG, "bad component, plane or null output");
   GD_HIP(c, hipStreamSynchronize(c->stream));
-  GD_HIP(c, hipMemcpy(out, c->f[c->cur][comp] + k * c->nx * c->ny, c->nx * c->ny * sizeof(float),
-                      hipMemcpyDeviceToHost));
+  const size_t P = c->nx * c->ny;
+  std::vector<unsigned char> buf(P * c->esz);
+  GD_HIP(c, hipMemcpy(buf.data(), static_cast<unsigned char*>(c->f[c->cur][comp]) + k * P * c->esz,
+                      P * c->esz, hipMemcpyDeviceToHost));
+  std::vector<double> v(P);
+  decode_to_f64(c, comp, buf.data(), v.data(), P);
+  for (size_t q = 0; q < P; ++q) out[q] = (float)v[q];
   return GD_OK;
 }
 
 int gd_reset(gd_ctx* c) {
   if (!c) return GD_ERR_ARG;
   for (int a = 0; a < 6; ++a)
-    GD_HIP(c, hipMemsetAsync(c->f[c->cur][a], 0, c->N * sizeof(float), c->stream));
+    GD_HIP(c, hipMemsetAsync(c->f[c->cur][a], 0, c->N * c->esz, c->stream));
   const size_t dn = (size_t)c->dft_ncomp * c->dft_f.size() * c->dft_plane;
   if (dn) {
     GD_HIP(c, hipMemsetAsync(c->dft_re, 0, dn * sizeof(double), c->stream));
@@ -459,10 +717,12 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
   size_t b = 0;
   for (int k = 0; k < 2; ++k)
     for (int a = 0; a < 6; ++a)
-      if (c->f[k][a]) b += c->N * sizeof(float);
+      if (c->f[k][a]) b += c->N * c->esz;
   for (int a = 0; a < 12; ++a)
-    if (c->coeff[a]) b += c->N * sizeof(float);
-  if (c->cb) b += c->N * sizeof(float);
+    if (c->coeff[a]) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
+  if (c->cb) b += c->N * c->esz;
+  if (c->mat_id) b += c->N + GD_MAX_MATERIALS * sizeof(float);
+  if (c->dft_snap) b += (size_t)gd::kMaxBlt * 6 * c->dft_plane * sizeof(float);
   for (auto& s : c->src) b += s.len * sizeof(float);
   b += 2 * (size_t)c->dft_ncomp * c->dft_f.size() * c->dft_plane * sizeof(double);
   b += c->dft_w_cap * sizeof(double);
@@ -493,9 +753,11 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       if (tiling->zchunk) t.zchunk = tiling->zchunk;
     }
     zt = gd::ZTiling{t.blt, t.rows, t.waves, t.zchunk};
-    if (!gd::ztiling_supported(zt, c->full))
+    if (c->storage == GD_STORAGE_F64)
+      return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel runs FP32 and FP16 storage only");
+    if (!gd::ztiling_supported(zt, (gd::Storage)c->storage, c->mat_kind))
       return c->fail(GD_ERR_UNSUPPORTED, "unsupported tiling (blt 1..4, rows x waves in "
-                                         "{2,4}x{4,8})");
+                                         "{2,4}x{4,8}, at least one owned row)");
     int rc = alloc_fields(c, 1);
     if (rc != GD_OK) return rc;
     if (c->dft_ncomp && !c->dft_snap) {
@@ -515,7 +777,12 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
   gd::Material m{};
   for (int a = 0; a < 12; ++a) m.c[a] = c->coeff[a];
   m.cb = c->cb;
+  m.id = c->mat_id;
+  m.table = c->mat_table;
   m.db = c->db;
+  m.db64 = (double)c->db;
+  const gd::Scales sc = scales_of(c);
+  const gd::Storage st = (gd::Storage)c->storage;
 
   gd::Sources s{};
   s.n = (int)c->src.size();
@@ -591,8 +858,8 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
         if (!a || !b) return c->fail(GD_ERR_HIP, "event creation failed");
         GD_HIP(c, hipEventRecord(a, c->stream));
       }
-      GD_HIP(c, gd::launch_plain_step(F, m, c->full, g, s, d, (int)(c->step + r), (int)r,
-                                      c->stream));
+      GD_HIP(c, gd::launch_plain_step(st, F, sc, m, c->mat_kind, g, s, d, (int)(c->step + r),
+                                      (int)r, c->stream));
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
     }
   } else {
@@ -607,8 +874,8 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
         if (!a || !b) return c->fail(GD_ERR_HIP, "event creation failed");
         GD_HIP(c, hipEventRecord(a, c->stream));
       }
-      GD_HIP(c, gd::launch_zstream(t, fields_of(c, c->cur), fields_of(c, 1 - c->cur), m, c->full,
-                                   g, s, d, (int)(c->step + r), (int)r, c->stream));
+      GD_HIP(c, gd::launch_zstream(t, st, fields_of(c, c->cur), fields_of(c, 1 - c->cur), sc, m,
+                                   c->mat_kind, g, s, d, (int)(c->step + r), (int)r, c->stream));
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
       if (d.ncomp) GD_HIP(c, gd::launch_dft_snapshot(d, t.blt, (int)r, c->stream));
       c->cur = 1 - c->cur;

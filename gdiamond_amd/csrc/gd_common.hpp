@@ -1,11 +1,13 @@
 // Structures shared by the host dispatcher and the gfx950 stepping kernels.
 //
-// Arithmetic contract (all kernels): FP32, every operation rounded once (the
+// Arithmetic contract (all kernels): every operation rounded once (the
 // library is compiled with -ffp-contract=off), operand order of the reference
 // update (gdiamond.hpp:1193-1221):
 //   E = Ca*E + Cb*(((Hp - Hq) - (Hr - Hs)) - J*dx)
 //   H = Da*H + Db*(((Ep - Eq) - (Er - Es)) - M*dx)
-// so results are bit-identical to the sequential CPU update.
+// computed in the storage format's compute type (FP32 for FP32 and FP16
+// storage, FP64 for FP64 storage), so results are bit-identical to the
+// sequential CPU update of the same storage format.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,18 +18,85 @@ namespace gd {
 constexpr int kMaxSources = 8;
 constexpr int kMaxBlt = 8;
 
-// Field arrays of one field copy (SoA, idx = i + j*nx + k*nx*ny).
+// Storage formats of the field copies (and of the isotropic coefficient).
+//   kF32: float, computed in float (the reference's format).
+//   kF64: double, computed in double (reference runs for error studies).
+//   kF16: IEEE binary16 of value * 2^-k_c per component c; values are
+//         decoded exactly to float, computed in float and rounded to binary16
+//         (round to nearest even) when stored: after every half step in the
+//         PLAIN kernels, once per launch of B steps in the TILED kernel.
+enum Storage { kF32 = 0, kF64 = 1, kF16 = 2 };
+
+// Field arrays of one field copy (SoA, idx = i + j*nx + k*nx*ny); the element
+// type follows the storage format.
 struct Fields {
-  float* c[6];  // Ex Ey Ez Hx Hy Hz
+  void* c[6];  // Ex Ey Ez Hx Hy Hz
 };
 
-// Material. Full: 12 reference arrays Cax Cbx Cay Cby Caz Cbz Dax Dbx Day Dby
-// Daz Dbz. Isotropic lossless: Ca = Da = 1 (multiplication by 1 is exact and
-// omitted), cb per cell, db scalar.
+// value = stored * s[c] for 16-bit storage (s = 2^k_c, exact), inv = 1/s.
+struct Scales {
+  float s[6];
+  float inv[6];
+};
+
+// Material kinds.
+//   kMatIso:  isotropic lossless, Ca = Da = 1 (multiplication by 1 is exact and
+//             omitted), cb per cell in the storage format, db scalar.
+//   kMatFull: the reference's 12 arrays Cax Cbx Cay Cby Caz Cbz Dax Dbx Day Dby
+//             Daz Dbz in the compute type.
+//   kMatIdx:  isotropic lossless with a 1-byte material id per cell and an FP32
+//             table of at most 256 Cb values (exact Cb, 1 byte per cell).
+enum MatKind { kMatIso = 0, kMatFull = 1, kMatIdx = 2 };
+constexpr int kMaxMaterials = 256;
+
 struct Material {
-  const float* c[12];
-  const float* cb;
+  const void* c[12];
+  const void* cb;
+  const uint8_t* id;     // kMatIdx: material id per cell
+  const float* table;    // kMatIdx: Cb of each id (device, kMaxMaterials entries)
   float db;
+  double db64;
+};
+
+// Decode / encode between a storage element and its compute type.
+template <class S>
+struct Codec;
+template <>
+struct Codec<float> {
+  using T = float;
+  __device__ __forceinline__ static T dec(float v, float) { return v; }
+  __device__ __forceinline__ static float enc(T v, float) { return v; }
+  // value after one store-load round trip
+  __device__ __forceinline__ static T round(T v, float, float) { return v; }
+};
+template <>
+struct Codec<double> {
+  using T = double;
+  __device__ __forceinline__ static T dec(double v, float) { return v; }
+  __device__ __forceinline__ static double enc(T v, float) { return v; }
+  __device__ __forceinline__ static T round(T v, float, float) { return v; }
+};
+// binary16 storage of value * 2^-k. The empty asm statements keep the scaling
+// multiply and the conversion apart: left alone, the compiler fuses them into
+// v_fma_mix* with a +0 addend, which turns a product that rounds to -0 into +0
+// (and a decoded -0 into +0), so results would differ from IEEE
+// multiply-then-convert in the sign of zero.
+template <>
+struct Codec<_Float16> {
+  using T = float;
+  __device__ __forceinline__ static T dec(_Float16 v, float s) {
+    float f = (float)v;
+    asm("" : "+v"(f));
+    return f * s;
+  }
+  __device__ __forceinline__ static _Float16 enc(T v, float inv) {
+    float t = v * inv;
+    asm("" : "+v"(t));
+    return (_Float16)t;
+  }
+  __device__ __forceinline__ static T round(T v, float s, float inv) {
+    return dec(enc(v, inv), s);
+  }
 };
 
 // Point sources. The value table[n] enters the component's update at step n.
@@ -75,9 +144,8 @@ __device__ __forceinline__ long long dft_plane_index(const Dft& d, const Grid& g
 }
 
 // re += v*wc; im -= v*ws for every frequency, for the component in `slot`.
-__device__ __forceinline__ void dft_accumulate(const Dft& d, int slot, long long p, float v,
+__device__ __forceinline__ void dft_accumulate(const Dft& d, int slot, long long p, double dv,
                                                int run_step) {
-  const double dv = (double)v;
   for (int f = 0; f < d.nf; ++f) {
     const long long o = ((long long)slot * d.nf + f) * d.plane + p;
     const double wc = d.wc[(long long)f * d.run_len + run_step];
@@ -91,8 +159,8 @@ __device__ __forceinline__ void dft_accumulate(const Dft& d, int slot, long long
 // v = Hx,Hy,Hz) of one cell. Kept out of line: it runs only on the monitored
 // plane, and inlining its scalar state would crowd the stepping loop.
 template <bool kE>
-__device__ __forceinline__ void dft_accumulate_group(const Dft& d, long long p, float v0,
-                                                               float v1, float v2, int run_step) {
+__device__ __forceinline__ void dft_accumulate_group(const Dft& d, long long p, double v0,
+                                                     double v1, double v2, int run_step) {
   const int base = kE ? 0 : 3;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -128,17 +196,18 @@ __device__ __forceinline__ void source_terms(const Sources& s, int base_comp, in
 
 // Host-side launchers (defined in the .hip kernel files).
 namespace gd {
-hipError_t launch_plain_step(const Fields& f, const Material& m, bool full, const Grid& g,
-                             const Sources& s, const Dft& d, int n, int run_step,
-                             hipStream_t stream);
+hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, const Material& m,
+                             MatKind mk, const Grid& g, const Sources& s, const Dft& d, int n,
+                             int run_step, hipStream_t stream);
 
 struct ZTiling {
   int blt, rows, waves, zchunk;
 };
-bool ztiling_supported(const ZTiling& t, bool full);
-hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst, const Material& m,
-                          bool full, const Grid& g, const Sources& s, const Dft& d, int n0,
-                          int run_step0, hipStream_t stream);
+bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk);
+hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
+                          const Scales& sc, const Material& m, MatKind mk, const Grid& g,
+                          const Sources& s, const Dft& d, int n0, int run_step0,
+                          hipStream_t stream);
 
 hipError_t launch_copy(const float* src, float* dst, size_t n, hipStream_t stream);
 

@@ -6,10 +6,15 @@
 // per step in update_FDTD_gpu_3D_warp_underutilization_fix,
 // gdiamond_gpu.cuh:896-913). The mapping is MI355X-first instead of one thread
 // per cell: a 64x4 workgroup covers a 64-wide x row segment (one wave64 per
-// row, 256 B coalesced per array access) and each thread marches along z over
-// a chunk of planes, reusing the z-neighbour it loaded on the previous plane
+// row, coalesced per array access) and each thread marches along z over a
+// chunk of planes, reusing the z-neighbour it loaded on the previous plane
 // from registers. The DFT monitor of E components is fused into the E kernel
 // (the value after step n is final there); H components into the H kernel.
+//
+// Templated on the storage element S (float, double or _Float16 with a
+// per-component power-of-two scale; arithmetic runs in Codec<S>::T) and on the
+// material kind (dense isotropic Cb, the reference's 12 arrays, or a 1-byte
+// material id into a small FP32 Cb table that stays in cache).
 #include "gd_common.hpp"
 
 namespace gd {
@@ -20,10 +25,12 @@ constexpr int kBx = 64;   // one wave64 along x
 constexpr int kBy = 4;    // rows per workgroup
 constexpr int kKc = 16;   // z planes per thread
 
-template <bool kFull>
-__global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Material m, Grid g,
+template <class S, int kMat>
+__global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Scales sc, Material m, Grid g,
                                                            Sources s, Dft d, int n,
                                                            int run_step) {
+  using C = Codec<S>;
+  using T = typename C::T;
   const int i = blockIdx.x * kBx + threadIdx.x;
   const int j = blockIdx.y * kBy + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -33,66 +40,82 @@ __global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Material m,
   const bool dft_e = (d.mask & 7u) != 0u;
   if (!ij_int && !dft_e) return;
 
-  float* __restrict__ Ex = f.c[0];
-  float* __restrict__ Ey = f.c[1];
-  float* __restrict__ Ez = f.c[2];
-  const float* __restrict__ Hx = f.c[3];
-  const float* __restrict__ Hy = f.c[4];
-  const float* __restrict__ Hz = f.c[5];
+  S* __restrict__ Ex = static_cast<S*>(f.c[0]);
+  S* __restrict__ Ey = static_cast<S*>(f.c[1]);
+  S* __restrict__ Ez = static_cast<S*>(f.c[2]);
+  const S* __restrict__ Hx = static_cast<const S*>(f.c[3]);
+  const S* __restrict__ Hy = static_cast<const S*>(f.c[4]);
+  const S* __restrict__ Hz = static_cast<const S*>(f.c[5]);
+  const T* const* cf = reinterpret_cast<const T* const*>(m.c);
+  const S* __restrict__ cbp = static_cast<const S*>(m.cb);
   const long long nx = g.nx, sxy = g.sxy;
+  const T dx = (T)g.dx;
   long long idx = (long long)i + (long long)j * nx + (long long)k0 * sxy;
+  auto hx_ = [&](long long q) { return C::dec(Hx[q], sc.s[3]); };
+  auto hy_ = [&](long long q) { return C::dec(Hy[q], sc.s[4]); };
+  auto hz_ = [&](long long q) { return C::dec(Hz[q], sc.s[5]); };
 
   // z-1 neighbours of the first plane; later ones come from the previous plane.
-  float hx_km = 0.0f, hy_km = 0.0f;
+  T hx_km = 0, hy_km = 0;
   if (ij_int && k0 >= 1) {
-    hx_km = Hx[idx - sxy];
-    hy_km = Hy[idx - sxy];
+    hx_km = hx_(idx - sxy);
+    hy_km = hy_(idx - sxy);
   }
   for (int k = k0; k < k1; ++k, idx += sxy) {
-    float ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
-    float hx = 0.0f, hy = 0.0f;
+    T ex = C::dec(Ex[idx], sc.s[0]), ey = C::dec(Ey[idx], sc.s[1]), ez = C::dec(Ez[idx], sc.s[2]);
+    T hx = 0, hy = 0;
     if (ij_int) {
-      hx = Hx[idx];
-      hy = Hy[idx];
+      hx = hx_(idx);
+      hy = hy_(idx);
       if (k >= 1 && k <= g.nz - 2) {
-        const float hz = Hz[idx];
-        const float hz_jm = Hz[idx - nx];
-        const float hz_im = Hz[idx - 1];
-        const float hy_im = Hy[idx - 1];
-        const float hx_jm = Hx[idx - nx];
+        const T hz = hz_(idx);
+        const T hz_jm = hz_(idx - nx);
+        const T hz_im = hz_(idx - 1);
+        const T hy_im = hy_(idx - 1);
+        const T hx_jm = hx_(idx - nx);
         float jx, jy, jz;
         source_terms(s, 0, i, j, k, n, jx, jy, jz);
-        const float cx = ((hz - hz_jm) - (hy - hy_km)) - jx * g.dx;
-        const float cy = ((hx - hx_km) - (hz - hz_im)) - jy * g.dx;
-        const float cz = ((hy - hy_im) - (hx - hx_jm)) - jz * g.dx;
-        if (kFull) {
-          ex = m.c[0][idx] * ex + m.c[1][idx] * cx;
-          ey = m.c[2][idx] * ey + m.c[3][idx] * cy;
-          ez = m.c[4][idx] * ez + m.c[5][idx] * cz;
+        const T cx = ((hz - hz_jm) - (hy - hy_km)) - (T)jx * dx;
+        const T cy = ((hx - hx_km) - (hz - hz_im)) - (T)jy * dx;
+        const T cz = ((hy - hy_im) - (hx - hx_jm)) - (T)jz * dx;
+        if constexpr (kMat == kMatFull) {
+          ex = cf[0][idx] * ex + cf[1][idx] * cx;
+          ey = cf[2][idx] * ey + cf[3][idx] * cy;
+          ez = cf[4][idx] * ez + cf[5][idx] * cz;
         } else {
-          const float b = m.cb[idx];
+          T b;
+          if constexpr (kMat == kMatIdx)
+            b = (T)m.table[m.id[idx]];
+          else
+            b = C::dec(cbp[idx], 1.0f);
           ex = ex + b * cx;
           ey = ey + b * cy;
           ez = ez + b * cz;
         }
-        Ex[idx] = ex;
-        Ey[idx] = ey;
-        Ez[idx] = ez;
+        Ex[idx] = C::enc(ex, sc.inv[0]);
+        Ey[idx] = C::enc(ey, sc.inv[1]);
+        Ez[idx] = C::enc(ez, sc.inv[2]);
+        // the field value after the step is the stored one
+        ex = C::round(ex, sc.s[0], sc.inv[0]);
+        ey = C::round(ey, sc.s[1], sc.inv[1]);
+        ez = C::round(ez, sc.s[2], sc.inv[2]);
       }
     }
     if (dft_e) {
       const long long p = dft_plane_index(d, g, i, j, k);
-      if (p >= 0) dft_accumulate_group<true>(d, p, ex, ey, ez, run_step);
+      if (p >= 0) dft_accumulate_group<true>(d, p, (double)ex, (double)ey, (double)ez, run_step);
     }
     hx_km = hx;
     hy_km = hy;
   }
 }
 
-template <bool kFull>
-__global__ __launch_bounds__(kBx* kBy) void plain_h_kernel(Fields f, Material m, Grid g,
+template <class S, int kMat>
+__global__ __launch_bounds__(kBx* kBy) void plain_h_kernel(Fields f, Scales sc, Material m, Grid g,
                                                            Sources s, Dft d, int n,
                                                            int run_step) {
+  using C = Codec<S>;
+  using T = typename C::T;
   const int i = blockIdx.x * kBx + threadIdx.x;
   const int j = blockIdx.y * kBy + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
@@ -102,58 +125,67 @@ __global__ __launch_bounds__(kBx* kBy) void plain_h_kernel(Fields f, Material m,
   const bool dft_h = (d.mask & 56u) != 0u;
   if (!ij_int && !dft_h) return;
 
-  const float* __restrict__ Ex = f.c[0];
-  const float* __restrict__ Ey = f.c[1];
-  const float* __restrict__ Ez = f.c[2];
-  float* __restrict__ Hx = f.c[3];
-  float* __restrict__ Hy = f.c[4];
-  float* __restrict__ Hz = f.c[5];
+  const S* __restrict__ Ex = static_cast<const S*>(f.c[0]);
+  const S* __restrict__ Ey = static_cast<const S*>(f.c[1]);
+  const S* __restrict__ Ez = static_cast<const S*>(f.c[2]);
+  S* __restrict__ Hx = static_cast<S*>(f.c[3]);
+  S* __restrict__ Hy = static_cast<S*>(f.c[4]);
+  S* __restrict__ Hz = static_cast<S*>(f.c[5]);
+  const T* const* cf = reinterpret_cast<const T* const*>(m.c);
   const long long nx = g.nx, sxy = g.sxy;
+  const T dx = (T)g.dx;
+  const T db = sizeof(T) == 8 ? (T)m.db64 : (T)m.db;
   long long idx = (long long)i + (long long)j * nx + (long long)k0 * sxy;
+  auto ex_ = [&](long long q) { return C::dec(Ex[q], sc.s[0]); };
+  auto ey_ = [&](long long q) { return C::dec(Ey[q], sc.s[1]); };
+  auto ez_ = [&](long long q) { return C::dec(Ez[q], sc.s[2]); };
 
   // own E of the current plane; the z+1 values loaded on a plane become the
   // own values of the next one.
-  float ex = 0.0f, ey = 0.0f;
+  T ex = 0, ey = 0;
   if (ij_int) {
-    ex = Ex[idx];
-    ey = Ey[idx];
+    ex = ex_(idx);
+    ey = ey_(idx);
   }
   for (int k = k0; k < k1; ++k, idx += sxy) {
-    float hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
-    float ex_kp = 0.0f, ey_kp = 0.0f;
+    T hx = C::dec(Hx[idx], sc.s[3]), hy = C::dec(Hy[idx], sc.s[4]), hz = C::dec(Hz[idx], sc.s[5]);
+    T ex_kp = 0, ey_kp = 0;
     if (ij_int) {
       if (k + 1 < g.nz) {
-        ex_kp = Ex[idx + sxy];
-        ey_kp = Ey[idx + sxy];
+        ex_kp = ex_(idx + sxy);
+        ey_kp = ey_(idx + sxy);
       }
       if (k >= 1 && k <= g.nz - 2) {
-        const float ez = Ez[idx];
-        const float ez_jp = Ez[idx + nx];
-        const float ez_ip = Ez[idx + 1];
-        const float ex_jp = Ex[idx + nx];
-        const float ey_ip = Ey[idx + 1];
+        const T ez = ez_(idx);
+        const T ez_jp = ez_(idx + nx);
+        const T ez_ip = ez_(idx + 1);
+        const T ex_jp = ex_(idx + nx);
+        const T ey_ip = ey_(idx + 1);
         float mx, my, mz;
         source_terms(s, 3, i, j, k, n, mx, my, mz);
-        const float cx = ((ey_kp - ey) - (ez_jp - ez)) - mx * g.dx;
-        const float cy = ((ez_ip - ez) - (ex_kp - ex)) - my * g.dx;
-        const float cz = ((ex_jp - ex) - (ey_ip - ey)) - mz * g.dx;
-        if (kFull) {
-          hx = m.c[6][idx] * hx + m.c[7][idx] * cx;
-          hy = m.c[8][idx] * hy + m.c[9][idx] * cy;
-          hz = m.c[10][idx] * hz + m.c[11][idx] * cz;
+        const T cx = ((ey_kp - ey) - (ez_jp - ez)) - (T)mx * dx;
+        const T cy = ((ez_ip - ez) - (ex_kp - ex)) - (T)my * dx;
+        const T cz = ((ex_jp - ex) - (ey_ip - ey)) - (T)mz * dx;
+        if constexpr (kMat == kMatFull) {
+          hx = cf[6][idx] * hx + cf[7][idx] * cx;
+          hy = cf[8][idx] * hy + cf[9][idx] * cy;
+          hz = cf[10][idx] * hz + cf[11][idx] * cz;
         } else {
-          hx = hx + m.db * cx;
-          hy = hy + m.db * cy;
-          hz = hz + m.db * cz;
+          hx = hx + db * cx;
+          hy = hy + db * cy;
+          hz = hz + db * cz;
         }
-        Hx[idx] = hx;
-        Hy[idx] = hy;
-        Hz[idx] = hz;
+        Hx[idx] = C::enc(hx, sc.inv[3]);
+        Hy[idx] = C::enc(hy, sc.inv[4]);
+        Hz[idx] = C::enc(hz, sc.inv[5]);
+        hx = C::round(hx, sc.s[3], sc.inv[3]);
+        hy = C::round(hy, sc.s[4], sc.inv[4]);
+        hz = C::round(hz, sc.s[5], sc.inv[5]);
       }
     }
     if (dft_h) {
       const long long p = dft_plane_index(d, g, i, j, k);
-      if (p >= 0) dft_accumulate_group<false>(d, p, hx, hy, hz, run_step);
+      if (p >= 0) dft_accumulate_group<false>(d, p, (double)hx, (double)hy, (double)hz, run_step);
     }
     ex = ex_kp;
     ey = ey_kp;
@@ -172,21 +204,38 @@ __global__ __launch_bounds__(256) void copy_kernel(const float* __restrict__ src
   }
 }
 
-}  // namespace
-
-hipError_t launch_plain_step(const Fields& f, const Material& m, bool full, const Grid& g,
-                             const Sources& s, const Dft& d, int n, int run_step,
-                             hipStream_t stream) {
+template <class S, int kMat>
+hipError_t launch_pair(const Fields& f, const Scales& sc, const Material& m, const Grid& g,
+                       const Sources& s, const Dft& d, int n, int run_step, hipStream_t stream) {
   const dim3 block(kBx, kBy, 1);
   const dim3 grid((g.nx + kBx - 1) / kBx, (g.ny + kBy - 1) / kBy, (g.nz + kKc - 1) / kKc);
-  if (full) {
-    hipLaunchKernelGGL(plain_e_kernel<true>, grid, block, 0, stream, f, m, g, s, d, n, run_step);
-    hipLaunchKernelGGL(plain_h_kernel<true>, grid, block, 0, stream, f, m, g, s, d, n, run_step);
-  } else {
-    hipLaunchKernelGGL(plain_e_kernel<false>, grid, block, 0, stream, f, m, g, s, d, n, run_step);
-    hipLaunchKernelGGL(plain_h_kernel<false>, grid, block, 0, stream, f, m, g, s, d, n, run_step);
-  }
+  hipLaunchKernelGGL((plain_e_kernel<S, kMat>), grid, block, 0, stream, f, sc, m, g, s, d, n,
+                     run_step);
+  hipLaunchKernelGGL((plain_h_kernel<S, kMat>), grid, block, 0, stream, f, sc, m, g, s, d, n,
+                     run_step);
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, const Material& m,
+                             MatKind mk, const Grid& g, const Sources& s, const Dft& d, int n,
+                             int run_step, hipStream_t stream) {
+  switch (st) {
+    case kF32:
+      if (mk == kMatFull) return launch_pair<float, kMatFull>(f, sc, m, g, s, d, n, run_step, stream);
+      if (mk == kMatIdx) return launch_pair<float, kMatIdx>(f, sc, m, g, s, d, n, run_step, stream);
+      return launch_pair<float, kMatIso>(f, sc, m, g, s, d, n, run_step, stream);
+    case kF64:
+      if (mk == kMatFull) return launch_pair<double, kMatFull>(f, sc, m, g, s, d, n, run_step, stream);
+      if (mk == kMatIdx) return hipErrorInvalidValue;
+      return launch_pair<double, kMatIso>(f, sc, m, g, s, d, n, run_step, stream);
+    case kF16:
+      if (mk == kMatFull) return hipErrorInvalidValue;
+      if (mk == kMatIdx) return launch_pair<_Float16, kMatIdx>(f, sc, m, g, s, d, n, run_step, stream);
+      return launch_pair<_Float16, kMatIso>(f, sc, m, g, s, d, n, run_step, stream);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_copy(const float* src, float* dst, size_t n, hipStream_t stream) {

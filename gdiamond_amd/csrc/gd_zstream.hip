@@ -28,6 +28,15 @@
 // (tile rim, planes below the chunk, columns outside the grid) hold garbage
 // that only ever feeds other cells outside the cone, so loads are clamped into
 // the grid instead of predicated.
+//
+// Storage formats: values are decoded when loaded and encoded when stored;
+// the B levels in between stay in FP32 registers. For binary16 storage a
+// launch therefore rounds to binary16 once per B steps (at its store), and the
+// DFT snapshots hold the FP32 value of every level. Every workgroup that
+// computes a cell at some level does so from the same decoded inputs with the
+// same operations, so overlapping tiles agree bit for bit.
+#include <type_traits>
+
 #include "gd_common.hpp"
 
 namespace gd {
@@ -37,6 +46,7 @@ namespace {
 struct ZArgs {
   Fields src;
   Fields dst;
+  Scales sc;
   Material m;
   Grid g;
   Sources s;
@@ -72,11 +82,13 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // Load / store at a per-plane base (uniform, SGPRs) plus a 32-bit byte offset
 // (per lane), so the hardware's scalar-base + vector-offset addressing is used
 // instead of 64-bit per-lane address arithmetic.
-__device__ __forceinline__ float ldp(const char* base, unsigned off) {
-  return *reinterpret_cast<const float*>(base + off);
+template <class S>
+__device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
+  return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
 }
-__device__ __forceinline__ void stp(char* base, unsigned off, float v) {
-  *reinterpret_cast<float*>(base + off) = v;
+template <class S>
+__device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
+  *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
 }
 
 // Subtract the point-source terms of step n on plane z from the curl
@@ -108,8 +120,13 @@ constexpr int zstream_waves_per_simd() {
   return R <= 2 ? 4 : 2;
 }
 
-template <int B, int R, int W, bool kFull>
+template <int B, int R, int W, int kMat, class S>
 __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(ZArgs a) {
+  using C = Codec<S>;
+  constexpr bool kFull = kMat == kMatFull;
+  constexpr bool kIdx = kMat == kMatIdx;
+  constexpr unsigned ES = sizeof(S);
+  constexpr unsigned kLogES = ES == 4 ? 2 : (ES == 2 ? 1 : 0);
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
   // lds_h[level-1][parity of p][wave][Hz,Hx][lane]: last row of H of a level,
@@ -118,6 +135,8 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
   // level, read by the previous wave one stream position later (j+1).
   __shared__ float lds_h[B][2][W][2][64];
   __shared__ float lds_e[B][2][W][2][64];
+  // Cb of each material id (indexed material only)
+  __shared__ float lds_cb[kIdx ? kMaxMaterials : 1];
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -128,7 +147,8 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
   const int tz = tyz / a.nty;
 
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-  const size_t plane_bytes = (size_t)a.g.sxy * sizeof(float);
+  const size_t plane_bytes = (size_t)a.g.sxy * ES;
+  const Scales& sc = a.sc;
   const float dx = a.g.dx;
 
   const int gx = tx * OX - B + lane;
@@ -146,7 +166,7 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
     upd_xy[r] = intx && gy >= 1 && gy <= ny - 2;
     const int ly = w * R + r;
     own[r] = ownx && ly >= B && ly < B + OY && gy < ny;
-    off[r] = (cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx) * 4u;
+    off[r] = (cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx) * ES;
     srow[r] = 0u;
     for (int q = 0; q < a.s.n; ++q) {
       const SourceDesc sd = a.s.d[q];
@@ -154,7 +174,7 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
     }
     any_src = any_src || srow[r] != 0u;
   }
-  const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * 4u;
+  const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * ES;
   // wave-uniform: does any lane of this wave hold a source column?
   const bool wave_src = __builtin_amdgcn_ballot_w64(any_src) != 0;
 
@@ -211,29 +231,35 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
     (&lds_h[0][0][0][0][0])[q] = 0.0f;
     (&lds_e[0][0][0][0][0])[q] = 0.0f;
   }
+  if constexpr (kIdx) {
+    for (int q = threadIdx.x; q < kMaxMaterials; q += 64 * W) lds_cb[q] = a.m.table[q];
+  }
 
   // Level-0 values of the next stream position, loaded one iteration ahead so
-  // their HBM latency overlaps the level updates of the current position.
-  // Planes >= nz are clamped to nz-1: they only feed cells outside the cone.
-  float pe[R][3], ph[R][3], pcb[R], phmz, phmx;
+  // their HBM latency overlaps the level updates of the current position
+  // (kept in the storage format; decoded when consumed). Planes >= nz are
+  // clamped to nz-1: they only feed cells outside the cone.
+  using MT = typename std::conditional<kIdx, uint8_t, S>::type;
+  S pe[R][3], ph[R][3], phmz, phmx;
+  MT pcb[R];
   auto prefetch = [&](int pp) {
     const size_t pz = (size_t)min(pp, nz - 1) * plane_bytes;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const char* be = reinterpret_cast<const char*>(a.src.c[c]) + pz;
-      const char* bh = reinterpret_cast<const char*>(a.src.c[3 + c]) + pz;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        pe[r][c] = ldp(be, off[r]);
-        ph[r][c] = ldp(bh, off[r]);
+        pe[r][c] = ldp<S>(a.src.c[c], pz, off[r]);
+        ph[r][c] = ldp<S>(a.src.c[3 + c], pz, off[r]);
       }
     }
-    phmz = ldp(reinterpret_cast<const char*>(a.src.c[5]) + pz, offm);
-    phmx = ldp(reinterpret_cast<const char*>(a.src.c[3]) + pz, offm);
-    if (!kFull) {
-      const char* bc = reinterpret_cast<const char*>(a.m.cb) + pz;
+    phmz = ldp<S>(a.src.c[5], pz, offm);
+    phmx = ldp<S>(a.src.c[3], pz, offm);
+    if constexpr (kIdx) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) pcb[r] = ldp(bc, off[r]);
+      for (int r = 0; r < R; ++r) pcb[r] = ldp<uint8_t>(a.m.id, pz >> kLogES, off[r] >> kLogES);
+    } else if constexpr (!kFull) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) pcb[r] = ldp<S>(a.m.cb, pz, off[r]);
     }
   };
   prefetch(p_begin);
@@ -246,17 +272,22 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        ec[r][c] = pe[r][c];
-        hc[r][c] = ph[r][c];
+        ec[r][c] = C::dec(pe[r][c], sc.s[c]);
+        hc[r][c] = C::dec(ph[r][c], sc.s[3 + c]);
       }
-    float hmz = phmz, hmx = phmx;  // H of the previous level on row gy0-1
+    float hmz = C::dec(phmz, sc.s[5]), hmx = C::dec(phmx, sc.s[3]);  // row gy0-1
     if (!kFull) {
 #pragma unroll
       for (int b = B - 1; b > 0; --b)
 #pragma unroll
         for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
 #pragma unroll
-      for (int r = 0; r < R; ++r) cbq[0][r] = pcb[r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kIdx)
+          cbq[0][r] = lds_cb[pcb[r]];
+        else
+          cbq[0][r] = C::dec(pcb[r], 1.0f);
+      }
     }
     prefetch(p + 1);
 
@@ -290,12 +321,12 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
           }
           if (kFull) {
             const unsigned o = off[r];
-            const float ca0 = ldp(reinterpret_cast<const char*>(a.m.c[0]) + oE, o);
-            const float cb0 = ldp(reinterpret_cast<const char*>(a.m.c[1]) + oE, o);
-            const float ca1 = ldp(reinterpret_cast<const char*>(a.m.c[2]) + oE, o);
-            const float cb1 = ldp(reinterpret_cast<const char*>(a.m.c[3]) + oE, o);
-            const float ca2 = ldp(reinterpret_cast<const char*>(a.m.c[4]) + oE, o);
-            const float cb2 = ldp(reinterpret_cast<const char*>(a.m.c[5]) + oE, o);
+            const float ca0 = ldp<float>(a.m.c[0], oE, o);
+            const float cb0 = ldp<float>(a.m.c[1], oE, o);
+            const float ca1 = ldp<float>(a.m.c[2], oE, o);
+            const float cb1 = ldp<float>(a.m.c[3], oE, o);
+            const float ca2 = ldp<float>(a.m.c[4], oE, o);
+            const float cb2 = ldp<float>(a.m.c[5], oE, o);
             en[r][0] = upd ? ca0 * ec[r][0] + cb0 * cx : ec[r][0];
             en[r][1] = upd ? ca1 * ec[r][1] + cb1 * cy : ec[r][1];
             en[r][2] = upd ? ca2 * ec[r][2] + cb2 * cz : ec[r][2];
@@ -335,12 +366,12 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
           }
           if (kFull) {
             const unsigned o = off[r];
-            const float da0 = ldp(reinterpret_cast<const char*>(a.m.c[6]) + oH, o);
-            const float db0 = ldp(reinterpret_cast<const char*>(a.m.c[7]) + oH, o);
-            const float da1 = ldp(reinterpret_cast<const char*>(a.m.c[8]) + oH, o);
-            const float db1 = ldp(reinterpret_cast<const char*>(a.m.c[9]) + oH, o);
-            const float da2 = ldp(reinterpret_cast<const char*>(a.m.c[10]) + oH, o);
-            const float db2 = ldp(reinterpret_cast<const char*>(a.m.c[11]) + oH, o);
+            const float da0 = ldp<float>(a.m.c[6], oH, o);
+            const float db0 = ldp<float>(a.m.c[7], oH, o);
+            const float da1 = ldp<float>(a.m.c[8], oH, o);
+            const float db1 = ldp<float>(a.m.c[9], oH, o);
+            const float da2 = ldp<float>(a.m.c[10], oH, o);
+            const float db2 = ldp<float>(a.m.c[11], oH, o);
             hn[r][0] = upd ? da0 * hq[b - 1][r][0] + db0 * cx : hq[b - 1][r][0];
             hn[r][1] = upd ? da1 * hq[b - 1][r][1] + db1 * cy : hq[b - 1][r][1];
             hn[r][2] = upd ? da2 * hq[b - 1][r][2] + db2 * cz : hq[b - 1][r][2];
@@ -387,28 +418,22 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
       if (b == B && own_any) {
         if (zE >= z_lo && zE < z_hi) {
           const size_t oE = (size_t)zE * plane_bytes;
-          char* d0 = reinterpret_cast<char*>(a.dst.c[0]) + oE;
-          char* d1 = reinterpret_cast<char*>(a.dst.c[1]) + oE;
-          char* d2 = reinterpret_cast<char*>(a.dst.c[2]) + oE;
 #pragma unroll
           for (int r = 0; r < R; ++r)
             if (own[r]) {
-              stp(d0, off[r], en[r][0]);
-              stp(d1, off[r], en[r][1]);
-              stp(d2, off[r], en[r][2]);
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                stp<S>(a.dst.c[c], oE, off[r], C::enc(en[r][c], sc.inv[c]));
             }
         }
         if (zH >= z_lo && zH < z_hi) {
           const size_t oH = (size_t)zH * plane_bytes;
-          char* d3 = reinterpret_cast<char*>(a.dst.c[3]) + oH;
-          char* d4 = reinterpret_cast<char*>(a.dst.c[4]) + oH;
-          char* d5 = reinterpret_cast<char*>(a.dst.c[5]) + oH;
 #pragma unroll
           for (int r = 0; r < R; ++r)
             if (own[r]) {
-              stp(d3, off[r], hn[r][0]);
-              stp(d4, off[r], hn[r][1]);
-              stp(d5, off[r], hn[r][2]);
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                stp<S>(a.dst.c[3 + c], oH, off[r], C::enc(hn[r][c], sc.inv[3 + c]));
             }
         }
       }
@@ -435,23 +460,37 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
   }
 }
 
-template <int B, int R, int W>
-hipError_t launch_one(const ZArgs& a, bool full, hipStream_t stream) {
+template <int B, int R, int W, int kMat, class S>
+void launch_k(const ZArgs& a, hipStream_t stream) {
   const unsigned blocks = (unsigned)((long long)a.ntx * a.nty * a.ntz);
-  if (full)
-    hipLaunchKernelGGL((zstream_kernel<B, R, W, true>), dim3(blocks), dim3(64 * W), 0, stream, a);
-  else
-    hipLaunchKernelGGL((zstream_kernel<B, R, W, false>), dim3(blocks), dim3(64 * W), 0, stream, a);
+  hipLaunchKernelGGL((zstream_kernel<B, R, W, kMat, S>), dim3(blocks), dim3(64 * W), 0, stream,
+                     a);
+}
+
+template <int B, int R, int W>
+hipError_t launch_one(const ZArgs& a, Storage st, MatKind mk, hipStream_t stream) {
+  if (st == kF16) {
+    if (mk == kMatIdx)
+      launch_k<B, R, W, kMatIdx, _Float16>(a, stream);
+    else
+      launch_k<B, R, W, kMatIso, _Float16>(a, stream);
+  } else if (mk == kMatFull) {
+    launch_k<B, R, W, kMatFull, float>(a, stream);
+  } else if (mk == kMatIdx) {
+    launch_k<B, R, W, kMatIdx, float>(a, stream);
+  } else {
+    launch_k<B, R, W, kMatIso, float>(a, stream);
+  }
   return hipGetLastError();
 }
 
 template <int R, int W>
-hipError_t launch_rw(int blt, const ZArgs& a, bool full, hipStream_t stream) {
+hipError_t launch_rw(int blt, const ZArgs& a, Storage st, MatKind mk, hipStream_t stream) {
   switch (blt) {
-    case 1: return launch_one<1, R, W>(a, full, stream);
-    case 2: return launch_one<2, R, W>(a, full, stream);
-    case 3: return launch_one<3, R, W>(a, full, stream);
-    case 4: return launch_one<4, R, W>(a, full, stream);
+    case 1: return launch_one<1, R, W>(a, st, mk, stream);
+    case 2: return launch_one<2, R, W>(a, st, mk, stream);
+    case 3: return launch_one<3, R, W>(a, st, mk, stream);
+    case 4: return launch_one<4, R, W>(a, st, mk, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -484,7 +523,8 @@ hipError_t launch_dft_snapshot(const Dft& d, int nlev, int run_step0, hipStream_
   return hipGetLastError();
 }
 
-bool ztiling_supported(const ZTiling& t, bool /*full*/) {
+bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk) {
+  if (st == kF64 || (st == kF16 && mk == kMatFull)) return false;
   if (t.blt < 1 || t.blt > 4 || t.zchunk < 1) return false;
   const bool rw = (t.rows == 2 && t.waves == 4) || (t.rows == 2 && t.waves == 8) ||
                   (t.rows == 4 && t.waves == 4) || (t.rows == 4 && t.waves == 8);
@@ -492,15 +532,17 @@ bool ztiling_supported(const ZTiling& t, bool /*full*/) {
   return t.rows * t.waves - 2 * t.blt >= 1 && 64 - 2 * t.blt >= 1;
 }
 
-hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst,
-                          const Material& m, bool full, const Grid& g, const Sources& s,
-                          const Dft& d, int n0, int run_step0, hipStream_t stream) {
-  if (!ztiling_supported(t, full)) return hipErrorInvalidValue;
+hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
+                          const Scales& sc, const Material& m, MatKind mk, const Grid& g,
+                          const Sources& s, const Dft& d, int n0, int run_step0,
+                          hipStream_t stream) {
+  if (!ztiling_supported(t, st, mk)) return hipErrorInvalidValue;
   // in-plane byte offsets are 32-bit
   if ((unsigned long long)g.sxy * 4ull >= (1ull << 32)) return hipErrorInvalidValue;
   ZArgs a;
   a.src = src;
   a.dst = dst;
+  a.sc = sc;
   a.m = m;
   a.g = g;
   a.s = s;
@@ -513,10 +555,10 @@ hipError_t launch_zstream(const ZTiling& t, const Fields& src, const Fields& dst
   a.nty = (g.ny + oy - 1) / oy;
   a.zc = t.zchunk;
   a.ntz = (g.nz + t.zchunk - 1) / t.zchunk;
-  if (t.rows == 2 && t.waves == 4) return launch_rw<2, 4>(t.blt, a, full, stream);
-  if (t.rows == 2 && t.waves == 8) return launch_rw<2, 8>(t.blt, a, full, stream);
-  if (t.rows == 4 && t.waves == 4) return launch_rw<4, 4>(t.blt, a, full, stream);
-  return launch_rw<4, 8>(t.blt, a, full, stream);
+  if (t.rows == 2 && t.waves == 4) return launch_rw<2, 4>(t.blt, a, st, mk, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_rw<2, 8>(t.blt, a, st, mk, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_rw<4, 4>(t.blt, a, st, mk, stream);
+  return launch_rw<4, 8>(t.blt, a, st, mk, stream);
 }
 
 }  // namespace gd

@@ -77,6 +77,13 @@ typedef struct gd_tiling {
 #define GD_MAX_BLT 8
 #define GD_MAX_SOURCES 8
 
+/* Field storage formats (gd_set_storage). */
+enum {
+  GD_STORAGE_F32 = 0, /* float fields, float arithmetic: the reference's format */
+  GD_STORAGE_F64 = 1, /* double fields and arithmetic (PLAIN only): reference runs */
+  GD_STORAGE_F16 = 2  /* binary16 of value*2^-k per component; float arithmetic */
+};
+
 /* ---- lifecycle --------------------------------------------------------- */
 
 /* Create a context for an nx*ny*nz grid (each >= 3) with cell size dx (used
@@ -88,7 +95,47 @@ const char* gd_last_error(const gd_ctx* ctx);
 /* Library version string, e.g. "gdiamond-hip 0.1 gfx950". */
 const char* gd_version(void);
 
+/* ---- storage ------------------------------------------------------------- */
+
+/* Storage format of the fields (and of the isotropic coefficient). Must be set
+ * before the material; resets the fields to zero and the step counter.
+ * GD_STORAGE_F16 stores each component c as binary16 of value * 2^-scale_exp[c]
+ * (scale_exp may be NULL: all 0). Values are decoded exactly to float,
+ * computed in float with the reference operand order and rounded to binary16
+ * (nearest even) when written to device memory: after each half step (E, then
+ * H) by GD_VARIANT_PLAIN, once per launch of `blt` steps by GD_VARIANT_TILED,
+ * which keeps the intermediate steps in FP32 registers (so TILED results
+ * depend on the tiling's blt, and a run's last launch covers the remaining
+ * nsteps % blt steps). The DFT accumulates the stored values (PLAIN) or the
+ * FP32 value of every step (TILED). It halves the field bytes per cell-update
+ * (24 B instead of 48 B; the material adds 1 B indexed, 2 B dense). No counterpart in the
+ * reference, which stores FP32 only (gdiamond.hpp:799-891). */
+int gd_set_storage(gd_ctx* ctx, int storage, const int* scale_exp);
+int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
+
 /* ---- material ------------------------------------------------------------ */
+
+/* Device formats of the isotropic material (gd_set_material_format), and what
+ * gd_get_material_format reports. */
+enum {
+  GD_MATERIAL_AUTO = 0,    /* INDEXED when Cb has at most GD_MAX_MATERIALS distinct values */
+  GD_MATERIAL_DENSE = 1,   /* one Cb per cell in the field storage format */
+  GD_MATERIAL_INDEXED = 2, /* one 1-byte material id per cell + an FP32 table of Cb values */
+  GD_MATERIAL_FULL = 3     /* reported only: the 12-array form of gd_set_coefficients */
+};
+#define GD_MAX_MATERIALS 256
+
+/* Choose how the isotropic Cb is held on the device; must precede the
+ * material (default GD_MATERIAL_AUTO). INDEXED keeps Cb exact in FP32 for
+ * every field storage format and costs 1 B per cell-update instead of 4 (FP32)
+ * or 2 (FP16); with GD_STORAGE_F16 the DENSE form stores Cb as binary16, whose
+ * rounding changes the medium (relative Cb error up to 2^-11). INDEXED is not
+ * available with GD_STORAGE_F64. No counterpart in the reference, which
+ * streams 12 FP32 coefficient arrays (gdiamond_gpu.cuh:871-887). */
+int gd_set_material_format(gd_ctx* ctx, int format);
+/* Format in use (DENSE, INDEXED or FULL; -1 before any material) and the
+ * number of table entries of the INDEXED form (0 otherwise). */
+int gd_get_material_format(const gd_ctx* ctx, int* format, int* ntable);
 
 /* Full reference coefficient arrays (12 arrays of nx*ny*nz floats), the layout
  * every reference driver copies to the device (gdiamond_gpu.cuh:871-887). */
@@ -99,6 +146,11 @@ int gd_set_isotropic(gd_ctx* ctx, const float* cb, float db);
 /* Isotropic form from relative permittivity: cb = dt/(eps_r*dx),
  * db = dt/dx, each evaluated in double and rounded to float once. */
 int gd_set_isotropic_eps(gd_ctx* ctx, const float* eps_r, float dt);
+/* Isotropic form by material id: cell q has Cb = table[id[q]] (ntable in
+ * 1..GD_MAX_MATERIALS, every id < ntable). Bit-identical to gd_set_isotropic
+ * with the expanded array. */
+int gd_set_isotropic_indexed(gd_ctx* ctx, const uint8_t* id, const float* table, int ntable,
+                             float db);
 
 /* ---- sources --------------------------------------------------------------- */
 
@@ -116,6 +168,9 @@ int gd_clear_sources(gd_ctx* ctx);
 
 int gd_set_fields(gd_ctx* ctx, const float* const fields[6]);
 int gd_get_fields(gd_ctx* ctx, float* const fields[6]);
+/* Exact field values in double for every storage format. */
+int gd_get_fields_f64(gd_ctx* ctx, double* const fields[6]);
+int gd_get_field_f64(gd_ctx* ctx, int component, double* out);
 /* Copy one field component, or one xy plane (k) of it, to the host. */
 int gd_get_field(gd_ctx* ctx, int component, float* out);
 int gd_get_plane_xy(gd_ctx* ctx, int component, size_t k, float* out);

@@ -185,6 +185,166 @@ DEFINE_DFT(dft_acc_f64, double)
 DEFINE_RUN(or_run_f32, float, dft_acc_f32)
 DEFINE_RUN(or_run_f64, double, dft_acc_f64)
 
+/* ------------------------------------------------------------------ */
+/* binary16 storage                                                     */
+/* ------------------------------------------------------------------ */
+
+/* Nearest binary16 by exact arithmetic: the spacing of binary16 numbers near
+ * |f| is q = 2^(e-11) for |f| >= 2^-14 (f = m*2^e, m in [0.5,1)) and 2^-24
+ * below; |f|/q is exact and rint() rounds it to nearest even. */
+uint16_t or_f32_to_f16(float f) {
+  const uint16_t sign = signbit(f) ? 0x8000u : 0u;
+  if (isnan(f)) return (uint16_t)(sign | 0x7e00u);
+  const double a = fabs((double)f);
+  if (a >= 65520.0) return (uint16_t)(sign | 0x7c00u);
+  if (a == 0.0) return sign;
+  int e;
+  frexp(a, &e);
+  const int qe = a >= ldexp(1.0, -14) ? e - 11 : -24;
+  const double n = rint(ldexp(a, -qe)); /* integer count of quanta */
+  if (qe == -24) return (uint16_t)(sign | (uint16_t)n); /* subnormal (1024 = min normal) */
+  int E = e - 1 + 15;
+  int mant = (int)n - 1024;
+  if ((int)n == 2048) { E += 1; mant = 0; }
+  if (E >= 31) return (uint16_t)(sign | 0x7c00u);
+  return (uint16_t)(sign | (E << 10) | mant);
+}
+
+float or_f16_to_f32(uint16_t h) {
+  const int E = (h >> 10) & 31, mant = h & 1023;
+  double v;
+  if (E == 31) v = mant ? NAN : INFINITY;
+  else if (E == 0) v = ldexp((double)mant, -24);
+  else v = ldexp((double)(mant + 1024), E - 25);
+  return (float)((h & 0x8000u) ? -v : v);
+}
+
+void or_f32_to_f16_n(const float* in, uint16_t* out, int64_t n) {
+  for (int64_t q = 0; q < n; ++q) out[q] = or_f32_to_f16(in[q]);
+}
+
+void or_f16_to_f32_n(const uint16_t* in, float* out, int64_t n) {
+  for (int64_t q = 0; q < n; ++q) out[q] = or_f16_to_f32(in[q]);
+}
+
+static inline float dec16(uint16_t h, float s) { return or_f16_to_f32(h) * s; }
+static inline uint16_t enc16(float v, float inv) { return or_f32_to_f16(v * inv); }
+
+static void dft_acc_f16(or_dft* d, int64_t nx, int64_t ny, int64_t nz, uint16_t* const f[6],
+                        const float* sc, int64_t n) {
+  if (!d || d->ncomp <= 0) return;
+  const int64_t P = plane_size(d, nx, ny, nz), sxy = nx * ny;
+  for (int c = 0; c < d->ncomp; ++c) {
+    const int comp = d->comps[c];
+    const uint16_t* a = f[comp];
+    for (int q = 0; q < d->nf; ++q) {
+      double wc, ws;
+      or_dft_weights(d->f_cps[q], n, d->scale, &wc, &ws);
+      double* re = d->re + ((int64_t)c * d->nf + q) * P;
+      double* im = d->im + ((int64_t)c * d->nf + q) * P;
+      for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+          for (int64_t i = 0; i < nx; ++i) {
+            int64_t p;
+            if (d->axis == 2) { if (k != d->index) continue; p = i + j * nx; }
+            else if (d->axis == 1) { if (j != d->index) continue; p = i + k * nx; }
+            else { if (i != d->index) continue; p = j + k * ny; }
+            const double e = (double)dec16(a[i + j * nx + k * sxy], sc[comp]);
+            re[p] += e * wc;
+            im[p] -= e * ws;
+          }
+    }
+  }
+}
+
+/* Block mode of or_run_f16: blocks of `block` steps (the last one shorter)
+ * run in float from the decoded state, then the state is rounded to binary16. */
+static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
+                          const float* sc, const float* inv, const float* cb, float db,
+                          int nsrc, const or_source* src, int64_t n0, int64_t nsteps,
+                          or_dft* dft, int64_t block) {
+  const int64_t N = nx * ny * nz;
+  float* W[6] = {0};
+  int rc = 0;
+  for (int c = 0; c < 6; ++c) {
+    W[c] = (float*)malloc((size_t)N * sizeof(float));
+    if (!W[c]) { rc = -2; goto done; }
+    for (int64_t q = 0; q < N; ++q) W[c][q] = dec16(F[c][q], sc[c]);
+  }
+  or_material m;
+  memset(&m, 0, sizeof(m));
+  m.full = 0;
+  m.cb = cb;
+  m.db = db;
+  for (int64_t n = n0; n < n0 + nsteps;) {
+    const int64_t nb = n0 + nsteps - n < block ? n0 + nsteps - n : block;
+    rc = or_run_f32(nx, ny, nz, dx, W, &m, nsrc, src, n, nb, dft, 1);
+    if (rc) goto done;
+    for (int c = 0; c < 6; ++c)
+      for (int64_t q = 0; q < N; ++q) {
+        F[c][q] = enc16(W[c][q], inv[c]);
+        W[c][q] = dec16(F[c][q], sc[c]);
+      }
+    n += nb;
+  }
+done:
+  for (int c = 0; c < 6; ++c) free(W[c]);
+  return rc;
+}
+
+int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
+               const int scale_exp[6], const float* cb, float db,
+               int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
+               int64_t block) {
+  if (nx < 3 || ny < 3 || nz < 3 || block < 0) return -1;
+  float sc[6], inv[6];
+  for (int c = 0; c < 6; ++c) {
+    sc[c] = ldexpf(1.0f, scale_exp ? scale_exp[c] : 0);
+    inv[c] = ldexpf(1.0f, scale_exp ? -scale_exp[c] : 0);
+  }
+  if (block > 0)
+    return run_f16_blocks(nx, ny, nz, dx, F, sc, inv, cb, db, nsrc, src, n0, nsteps, dft, block);
+  uint16_t *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
+  const int64_t sxy = nx * ny;
+  for (int64_t n = n0; n < n0 + nsteps; ++n) {
+    for (int64_t k = 1; k < nz - 1; k++)
+      for (int64_t j = 1; j < ny - 1; j++)
+        for (int64_t i = 1; i < nx - 1; i++) {
+          const int64_t idx = i + j * nx + k * sxy;
+          const float hz = dec16(Hz[idx], sc[5]), hy = dec16(Hy[idx], sc[4]),
+                      hx = dec16(Hx[idx], sc[3]);
+          const float cx = ((hz - dec16(Hz[idx - nx], sc[5])) - (hy - dec16(Hy[idx - sxy], sc[4]))) -
+                           src_val_f(nsrc, src, 0, i, j, k, n) * dx;
+          const float cy = ((hx - dec16(Hx[idx - sxy], sc[3])) - (hz - dec16(Hz[idx - 1], sc[5]))) -
+                           src_val_f(nsrc, src, 1, i, j, k, n) * dx;
+          const float cz = ((hy - dec16(Hy[idx - 1], sc[4])) - (hx - dec16(Hx[idx - nx], sc[3]))) -
+                           src_val_f(nsrc, src, 2, i, j, k, n) * dx;
+          const float b = cb[idx];
+          Ex[idx] = enc16(dec16(Ex[idx], sc[0]) + b * cx, inv[0]);
+          Ey[idx] = enc16(dec16(Ey[idx], sc[1]) + b * cy, inv[1]);
+          Ez[idx] = enc16(dec16(Ez[idx], sc[2]) + b * cz, inv[2]);
+        }
+    for (int64_t k = 1; k < nz - 1; k++)
+      for (int64_t j = 1; j < ny - 1; j++)
+        for (int64_t i = 1; i < nx - 1; i++) {
+          const int64_t idx = i + j * nx + k * sxy;
+          const float ex = dec16(Ex[idx], sc[0]), ey = dec16(Ey[idx], sc[1]),
+                      ez = dec16(Ez[idx], sc[2]);
+          const float cx = ((dec16(Ey[idx + sxy], sc[1]) - ey) - (dec16(Ez[idx + nx], sc[2]) - ez)) -
+                           src_val_f(nsrc, src, 3, i, j, k, n) * dx;
+          const float cy = ((dec16(Ez[idx + 1], sc[2]) - ez) - (dec16(Ex[idx + sxy], sc[0]) - ex)) -
+                           src_val_f(nsrc, src, 4, i, j, k, n) * dx;
+          const float cz = ((dec16(Ex[idx + nx], sc[0]) - ex) - (dec16(Ey[idx + 1], sc[1]) - ey)) -
+                           src_val_f(nsrc, src, 5, i, j, k, n) * dx;
+          Hx[idx] = enc16(dec16(Hx[idx], sc[3]) + db * cx, inv[3]);
+          Hy[idx] = enc16(dec16(Hy[idx], sc[4]) + db * cy, inv[4]);
+          Hz[idx] = enc16(dec16(Hz[idx], sc[5]) + db * cz, inv[5]);
+        }
+    dft_acc_f16(dft, nx, ny, nz, F, sc, n);
+  }
+  return 0;
+}
+
 /* Literal reference loop with J/M arrays (gdiamond.hpp:1164-1221): the arrays
  * are zero, each step writes the source values at their cells and every cell
  * subtracts J[idx]*dx. */

@@ -69,6 +69,30 @@ int or_run_f64(int64_t nx, int64_t ny, int64_t nz, double dx,
                int nsrc, const or_source* src,
                int64_t n0, int64_t nsteps, or_dft* dft, int nthreads);
 
+/* FP16-storage twin: every component c is stored as binary16 of
+ * value * 2^-scale_exp[c]; values are decoded exactly to float, computed in
+ * float with the operand order of or_run_f32 (isotropic Cb given in float:
+ * exact table values, or decoded binary16 for the dense 16-bit coefficient)
+ * and rounded to binary16 (nearest even) when they are stored:
+ *   block == 0: after every half step (E, then H), as an untiled update that
+ *               writes E and H to memory each step does; the DFT sees the
+ *               stored values;
+ *   block >= 1: after every `block` steps (the last block of the call may be
+ *               shorter), as a temporally blocked update that keeps the
+ *               intermediate steps in float registers; the DFT sees the float
+ *               value of every step (before the block-end rounding).
+ * Fields hold binary16 bit patterns. Isotropic material only. */
+int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const fields[6],
+               const int scale_exp[6], const float* cb, float db,
+               int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
+               int64_t block);
+
+/* binary16 conversions used by or_run_f16 (round to nearest even). */
+uint16_t or_f32_to_f16(float f);
+float or_f16_to_f32(uint16_t h);
+void or_f32_to_f16_n(const float* in, uint16_t* out, int64_t n);
+void or_f16_to_f32_n(const uint16_t* in, float* out, int64_t n);
+
 /* Literal form of the reference loop with full J/M arrays (6 arrays, zero
  * except where set per step), gdiamond.hpp:1164-1221. Used to pin the
  * point-source shortcut of or_run_f32. */

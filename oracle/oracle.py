@@ -78,6 +78,20 @@ def lib():
                                                              C.POINTER(C.c_float) * 12, C.c_int,
                                                              C.POINTER(_Source), C.c_int64,
                                                              C.c_int64]
+        L.or_run_f16.restype = C.c_int
+        L.or_run_f16.argtypes = [C.c_int64] * 3 + [C.c_float, C.POINTER(C.c_uint16) * 6,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                                   C.c_float, C.c_int, C.POINTER(_Source),
+                                                   C.c_int64, C.c_int64, C.POINTER(_Dft),
+                                                   C.c_int64]
+        L.or_f32_to_f16.restype = C.c_uint16
+        L.or_f32_to_f16.argtypes = [C.c_float]
+        L.or_f16_to_f32.restype = C.c_float
+        L.or_f16_to_f32.argtypes = [C.c_uint16]
+        L.or_f32_to_f16_n.restype = None
+        L.or_f32_to_f16_n.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint16), C.c_int64]
+        L.or_f16_to_f32_n.restype = None
+        L.or_f16_to_f32_n.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_int64]
         L.or_default_problem_coefficients.restype = None
         L.or_default_problem_coefficients.argtypes = [C.c_int64] * 3 + [C.POINTER(C.c_float) * 12]
         L.or_default_mz.restype = C.c_float
@@ -179,6 +193,87 @@ def run(shape, fields, *, coeffs=None, cb=None, db=None, dx=1.0, sources=(), n0=
     if rc != 0:
         raise RuntimeError(f"oracle run failed: {rc}")
     return out
+
+
+def _dft_struct(dft, shape, keep):
+    nx, ny, nz = shape
+    comps = list(dft["comps"])
+    f_cps = np.ascontiguousarray(dft["f_cps"], dtype=np.float64)
+    axis = int(dft["axis"])
+    plane = {0: ny * nz, 1: nx * nz, 2: nx * ny}[axis]
+    re = dft.get("re")
+    im = dft.get("im")
+    if re is None:
+        re = np.zeros((len(comps), len(f_cps), plane), np.float64)
+        im = np.zeros_like(re)
+    d = _Dft()
+    d.ncomp = len(comps)
+    for a, c in enumerate(comps):
+        d.comps[a] = c
+    d.axis = axis
+    d.index = int(dft["index"])
+    d.nf = len(f_cps)
+    d.f_cps = _dp(f_cps)
+    d.scale = float(dft.get("scale", 1.0))
+    d.re = _dp(re)
+    d.im = _dp(im)
+    keep += [f_cps, re, im, d]
+    return C.pointer(d), (re, im)
+
+
+def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps=1,
+            dft=None, block=0):
+    """FP16-storage twin: fields16 are 6 uint16 (binary16 bit) arrays shaped
+    (nz, ny, nx), advanced in place; cb the float32 Cb array (exact values, or
+    the decoded binary16 coefficient of the dense 16-bit form). ``block`` 0
+    rounds after every half step (untiled update), block B after every B steps
+    (temporally blocked update with B levels per launch)."""
+    nx, ny, nz = shape
+    sh = (nz, ny, nx)
+    for f in fields16:
+        _check_arr(f, sh, np.uint16)
+    keep = []
+    cb = np.ascontiguousarray(cb, dtype=np.float32).reshape(sh)
+    keep.append(cb)
+    se = (C.c_int * 6)(*[int(v) for v in scale_exp])
+    nsrc = len(sources)
+    src_arr = (_Source * max(nsrc, 1))()
+    for s, (comp, i, j, k, table) in enumerate(sources):
+        t = np.ascontiguousarray(table, dtype=np.float32)
+        keep.append(t)
+        src_arr[s].comp, src_arr[s].i, src_arr[s].j, src_arr[s].k = comp, i, j, k
+        src_arr[s].table = _fp(t)
+    dptr, out = (None, None)
+    if dft is not None:
+        dptr, out = _dft_struct(dft, shape, keep)
+    P16 = C.POINTER(C.c_uint16)
+    fa = (P16 * 6)(*[f.ctypes.data_as(P16) for f in fields16])
+    rc = lib().or_run_f16(nx, ny, nz, float(dx), fa, se, _fp(cb), float(db),
+                          nsrc, src_arr, n0, nsteps, dptr, int(block))
+    if rc != 0:
+        raise RuntimeError("oracle run failed")
+    return out
+
+
+def f32_to_f16_bits(values):
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    out = np.empty(v.size, np.uint16)
+    lib().or_f32_to_f16_n(_fp(v), out.ctypes.data_as(C.POINTER(C.c_uint16)), v.size)
+    return out
+
+
+def encode_f16(values, scale_exp=0):
+    """Storage encoding of float values: binary16 of value * 2^-scale_exp."""
+    v = np.ascontiguousarray(values, np.float32) * np.float32(2.0 ** -scale_exp)
+    return f32_to_f16_bits(v).reshape(np.shape(values))
+
+
+def decode_f16(bits, scale_exp=0):
+    b = np.ascontiguousarray(bits, np.uint16)
+    flat = np.ascontiguousarray(b.ravel())
+    out = np.empty(flat.size, np.float32)
+    lib().or_f16_to_f32_n(flat.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(out), flat.size)
+    return (out * np.float32(2.0 ** scale_exp)).reshape(b.shape)
 
 
 def run_jm_arrays(shape, fields, coeffs, dx, sources, n0, nsteps):

@@ -66,3 +66,11 @@ def describe_diff(got, want):
     q = bad[vals_equal[0]] if vals_equal.size else bad[0]
     return (f"{bad.size} bit-differences, {vals_equal.size} value-differences, "
             f"max |d| {np.abs(g - w).max():.3e}, first at flat {q}: got {g[q]!r} want {w[q]!r}")
+
+
+def spectral_error(re, im, ref_re, ref_im):
+    """Relative DFT-spectrum error ||X - X_ref||_2 / ||X_ref||_2 over the
+    monitor plane, per (component, frequency); arrays shaped (ncomp, nf, plane)."""
+    X = np.asarray(re, np.float64) + 1j * np.asarray(im, np.float64)
+    R = np.asarray(ref_re, np.float64) + 1j * np.asarray(ref_im, np.float64)
+    return np.linalg.norm(X - R, axis=-1) / np.linalg.norm(R, axis=-1)
