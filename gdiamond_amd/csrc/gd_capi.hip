@@ -429,7 +429,12 @@ int gd_get_material_format(const gd_ctx* c, int* format, int* ntable) {
 int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (!c) return GD_ERR_ARG;
   if (!cb) return c->fail(GD_ERR_ARG, "null cb array");
-  if (c->mat_req != GD_MATERIAL_DENSE) {
+  // AUTO: indexed for 16-bit storage (exact FP32 Cb instead of binary16),
+  // dense FP32 Cb otherwise (the same results; the dense tiled kernel is the
+  // faster one for FP32 fields)
+  const bool want_index = c->mat_req == GD_MATERIAL_INDEXED ||
+                          (c->mat_req == GD_MATERIAL_AUTO && c->storage == GD_STORAGE_F16);
+  if (want_index) {
     if (c->storage == GD_STORAGE_F64) {
       if (c->mat_req == GD_MATERIAL_INDEXED)
         return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or FP16 storage");

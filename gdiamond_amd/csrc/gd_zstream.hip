@@ -92,24 +92,23 @@ __device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, 
 }
 
 // Subtract the point-source terms of step n on plane z from the curl
-// brackets c of components base..base+2, for the sources flagged in `mask`
-// (sources in this column; at most one per cell and component). Out of line:
-// only the few lanes holding a source column ever call it.
-__device__ __forceinline__ float3 apply_sources(const Sources s, unsigned mask, int base,
-                                                          int z, int n, float dx, float3 c) {
-  while (mask) {
-    const int q = __builtin_ctz(mask);
-    mask &= mask - 1u;
+// brackets c[0..2] (components base..base+2) of row gy (uniform) at column gx.
+// The loop over sources is uniform; a lane applies a term only on
+// its own cell (at most one source per cell and component, so the order of
+// the subtractions does not matter). Called only by waves that hold a source
+// column.
+__device__ __forceinline__ void apply_sources_row(const Sources& s, int gx, int gy, int z, int n,
+                                                  float dx, int base, float (&c)[3]) {
+  for (int q = 0; q < s.n; ++q) {
     const SourceDesc sd = s.d[q];
-    if (sd.k != z) continue;
     const int k = sd.comp - base;
-    if (k < 0 || k > 2) continue;
+    if (sd.k != z || sd.j != gy || k < 0 || k > 2) continue;
     const float v = sd.table[n];
-    if (k == 0) c.x = c.x - v * dx;
-    if (k == 1) c.y = c.y - v * dx;
-    if (k == 2) c.z = c.z - v * dx;
+    const bool m = sd.i == gx;
+    c[0] = (m && k == 0) ? c[0] - v * dx : c[0];
+    c[1] = (m && k == 1) ? c[1] - v * dx : c[1];
+    c[2] = (m && k == 2) ? c[2] - v * dx : c[2];
   }
-  return c;
 }
 
 // Occupancy target per SIMD: the register state is ~6 values per row per time
@@ -158,7 +157,6 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
   const unsigned cgx = (unsigned)clampi(gx, 0, nx - 1);
   bool upd_xy[R], own[R];
   unsigned off[R];   // in-plane byte offset, clamped into the grid
-  unsigned srow[R];  // point sources in this column (bit q = source q)
   bool any_src = false;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -167,12 +165,10 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
     const int ly = w * R + r;
     own[r] = ownx && ly >= B && ly < B + OY && gy < ny;
     off[r] = (cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx) * ES;
-    srow[r] = 0u;
     for (int q = 0; q < a.s.n; ++q) {
       const SourceDesc sd = a.s.d[q];
-      if (sd.i == gx && sd.j == gy) srow[r] |= 1u << q;
+      any_src = any_src || (sd.i == gx && sd.j == gy);
     }
-    any_src = any_src || srow[r] != 0u;
   }
   const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * ES;
   // wave-uniform: does any lane of this wave hold a source column?
@@ -190,27 +186,37 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
   const bool dft_on = a.d.ncomp > 0;
   const int dft_zmul = a.d.axis == 2 ? 0 : (a.d.axis == 0 ? ny : nx);
   bool own_any = false, dft_any = false;
-  bool dft_row[R];
-  int dft_base[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int gy = gy0 + r;
     own_any = own_any || own[r];
     const bool on = a.d.axis == 2 || (a.d.axis == 0 ? gx == a.d.index : gy == a.d.index);
-    dft_row[r] = dft_on && own[r] && on;
-    dft_base[r] = a.d.axis == 2 ? gx + gy * nx : (a.d.axis == 0 ? gy : gx);
-    dft_any = dft_any || dft_row[r];
+    dft_any = dft_any || (dft_on && own[r] && on);
   }
   // wave-uniform: does this wave own any cell of the monitored plane?
   const bool wave_dft = __builtin_amdgcn_ballot_w64(dft_any) != 0;
+  // per-row monitor predicate and plane index base, evaluated only by waves
+  // that own monitored cells
+  auto dft_row = [&](int r) {
+    const int gy = gy0 + r;
+    return own[r] && (a.d.axis == 2 || (a.d.axis == 0 ? gx == a.d.index : gy == a.d.index));
+  };
+  auto dft_base = [&](int r) {
+    const int gy = gy0 + r;
+    return a.d.axis == 2 ? gx + gy * nx : (a.d.axis == 0 ? gy : gx);
+  };
 
   // Pipeline state (registers):
   //   eq[b] = E of level b on plane p-b   (b = 1..B, slot 0 unused)
   //   hq[b] = H of level b on plane p-b-1 (b = 0..B-1; level 0 = loaded H)
-  //   cbq[b-1] = Cb on plane p-b+1, used by level b (isotropic material)
+  //   cbq[b-1] = Cb on plane p-b+1, used by level b (dense isotropic material)
+  //   idq: material ids of planes p..p-B+1, one byte per level (byte b-1 for
+  //        level b), looked up in lds_cb per level (indexed material)
   float eq[B + 1][R][3];
   float hq[B][R][3];
-  float cbq[B][R];
+  float cbq[kIdx ? 1 : B][R];
+  unsigned idq[R];
+  static_assert(B <= 4, "material ids of B levels are packed in 32 bits");
 #pragma unroll
   for (int b = 0; b <= B; ++b)
 #pragma unroll
@@ -221,9 +227,11 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
         if (b < B) hq[b][r][c] = 0.0f;
       }
 #pragma unroll
-  for (int b = 0; b < B; ++b)
+  for (int b = 0; b < (kIdx ? 1 : B); ++b)
 #pragma unroll
     for (int r = 0; r < R; ++r) cbq[b][r] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < R; ++r) idq[r] = 0u;
 
   // Exchange rows start at zero; they are only consumed by cells outside the
   // valid cone before the first real write, but stay deterministic.
@@ -276,18 +284,16 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
         hc[r][c] = C::dec(ph[r][c], sc.s[3 + c]);
       }
     float hmz = C::dec(phmz, sc.s[5]), hmx = C::dec(phmx, sc.s[3]);  // row gy0-1
-    if (!kFull) {
+    if constexpr (kIdx) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) idq[r] = (idq[r] << 8) | (unsigned)pcb[r];
+    } else if constexpr (!kFull) {
 #pragma unroll
       for (int b = B - 1; b > 0; --b)
 #pragma unroll
         for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (kIdx)
-          cbq[0][r] = lds_cb[pcb[r]];
-        else
-          cbq[0][r] = C::dec(pcb[r], 1.0f);
-      }
+      for (int r = 0; r < R; ++r) cbq[0][r] = C::dec(pcb[r], 1.0f);
     }
     prefetch(p + 1);
 
@@ -309,16 +315,13 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
           const float hy_xm = from_lane_below(hc[r][1]);
           const float hz_ym = r > 0 ? hc[r - 1][2] : hmz;
           const float hx_ym = r > 0 ? hc[r - 1][0] : hmx;
+          float cr[3];
+          cr[0] = (hc[r][2] - hz_ym) - (hc[r][1] - hq[b - 1][r][1]);
+          cr[1] = (hc[r][0] - hq[b - 1][r][0]) - (hc[r][2] - hz_xm);
+          cr[2] = (hc[r][1] - hy_xm) - (hc[r][0] - hx_ym);
+          if (wave_src) apply_sources_row(a.s, gx, gy0 + r, zE, n, dx, 0, cr);
           const bool upd = upd_xy[r] && intzE;
-          float cx = (hc[r][2] - hz_ym) - (hc[r][1] - hq[b - 1][r][1]);
-          float cy = (hc[r][0] - hq[b - 1][r][0]) - (hc[r][2] - hz_xm);
-          float cz = (hc[r][1] - hy_xm) - (hc[r][0] - hx_ym);
-          if (wave_src && srow[r]) {
-            const float3 t = apply_sources(a.s, srow[r], 0, zE, n, dx, make_float3(cx, cy, cz));
-            cx = t.x;
-            cy = t.y;
-            cz = t.z;
-          }
+          const float cx = cr[0], cy = cr[1], cz = cr[2];
           if (kFull) {
             const unsigned o = off[r];
             const float ca0 = ldp<float>(a.m.c[0], oE, o);
@@ -331,7 +334,11 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
             en[r][1] = upd ? ca1 * ec[r][1] + cb1 * cy : ec[r][1];
             en[r][2] = upd ? ca2 * ec[r][2] + cb2 * cz : ec[r][2];
           } else {
-            const float cb = cbq[b - 1][r];
+            float cb;
+            if constexpr (kIdx)
+              cb = lds_cb[(idq[r] >> (8 * (b - 1))) & 0xffu];
+            else
+              cb = cbq[b - 1][r];
             en[r][0] = upd ? ec[r][0] + cb * cx : ec[r][0];
             en[r][1] = upd ? ec[r][1] + cb * cy : ec[r][1];
             en[r][2] = upd ? ec[r][2] + cb * cz : ec[r][2];
@@ -354,16 +361,13 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
           const float ey_xp = from_lane_above(eq[b][r][1]);
           const float ez_yp = r + 1 < R ? eq[b][r + 1][2] : ez_last;
           const float ex_yp = r + 1 < R ? eq[b][r + 1][0] : ex_last;
+          float cr[3];
+          cr[0] = (en[r][1] - eq[b][r][1]) - (ez_yp - eq[b][r][2]);
+          cr[1] = (ez_xp - eq[b][r][2]) - (en[r][0] - eq[b][r][0]);
+          cr[2] = (ex_yp - eq[b][r][0]) - (ey_xp - eq[b][r][1]);
+          if (wave_src) apply_sources_row(a.s, gx, gy0 + r, zH, n, dx, 3, cr);
           const bool upd = upd_xy[r] && intzH;
-          float cx = (en[r][1] - eq[b][r][1]) - (ez_yp - eq[b][r][2]);
-          float cy = (ez_xp - eq[b][r][2]) - (en[r][0] - eq[b][r][0]);
-          float cz = (ex_yp - eq[b][r][0]) - (ey_xp - eq[b][r][1]);
-          if (wave_src && srow[r]) {
-            const float3 t = apply_sources(a.s, srow[r], 3, zH, n, dx, make_float3(cx, cy, cz));
-            cx = t.x;
-            cy = t.y;
-            cz = t.z;
-          }
+          const float cx = cr[0], cy = cr[1], cz = cr[2];
           if (kFull) {
             const unsigned o = off[r];
             const float da0 = ldp<float>(a.m.c[6], oH, o);
@@ -395,13 +399,13 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
             float* sp = a.d.snap + ((size_t)(b - 1) * 6 + c) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              if (dft_row[r]) sp[dft_base[r] + zE * dft_zmul] = en[r][c];
+              if (dft_row(r)) sp[dft_base(r) + zE * dft_zmul] = en[r][c];
           }
           if (okH && ((a.d.mask >> (3 + c)) & 1u)) {
             float* sp = a.d.snap + ((size_t)(b - 1) * 6 + 3 + c) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              if (dft_row[r]) sp[dft_base[r] + zH * dft_zmul] = hn[r][c];
+              if (dft_row(r)) sp[dft_base(r) + zH * dft_zmul] = hn[r][c];
           }
         }
       }

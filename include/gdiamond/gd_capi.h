@@ -118,7 +118,8 @@ int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
 /* Device formats of the isotropic material (gd_set_material_format), and what
  * gd_get_material_format reports. */
 enum {
-  GD_MATERIAL_AUTO = 0,    /* INDEXED when Cb has at most GD_MAX_MATERIALS distinct values */
+  GD_MATERIAL_AUTO = 0,    /* GD_STORAGE_F16: INDEXED when Cb has at most GD_MAX_MATERIALS
+                              distinct values, else DENSE; other storage formats: DENSE */
   GD_MATERIAL_DENSE = 1,   /* one Cb per cell in the field storage format */
   GD_MATERIAL_INDEXED = 2, /* one 1-byte material id per cell + an FP32 table of Cb values */
   GD_MATERIAL_FULL = 3     /* reported only: the 12-array form of gd_set_coefficients */

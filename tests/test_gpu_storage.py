@@ -7,8 +7,8 @@ fixed-error comparison), and the dense / indexed isotropic material.
   (decode exactly, compute in float in the reference operand order, round to
   binary16 once per step) is pinned on the CPU against numpy's float16 in
   test_oracle_f16.py.
-* FP32 with the dense material (FP32 Cb per cell) == the oracle on every
-  tiling (the indexed form is what the other parity tests run by default).
+* FP32 with the dense (FP32 Cb per cell, what the other parity tests run by
+  default) and the indexed material == the oracle on every tiling.
 * FP64 PLAIN == oracle.run(dtype=float64) bit for bit (fields and DFT).
 * Material-format selection and errors.
 * Error of FP32 and FP16 storage against FP64 on the same problem.
@@ -124,20 +124,21 @@ def test_f16_tiled_matches_oracle(gd, oracle, tiling, mat):
     _assert_same(got, want, f"f16 tiled {tiling} {mat}")
 
 
+@pytest.mark.parametrize("mat", ["dense", "indexed"])
 @pytest.mark.parametrize("tiling", [None] + TILINGS,
                          ids=lambda t: "plain" if t is None else
                          "b{blt}r{rows}w{waves}z{zchunk}".format(**t))
-def test_f32_dense_material_matches_oracle(gd, oracle, tiling):
+def test_f32_material_matches_oracle(gd, oracle, tiling, mat):
     shape = (70, 45, 31)
     T = 24
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=6)
     init = random_fields(shape, 4, scale=0.05, boundary_zero=True)
     wf = [a.copy() for a in init]
     wre, wim = oracle.run(shape, wf, cb=cb, db=db, sources=src, nsteps=T, dft=dft)
-    say("f32 dense", shape, tiling)
+    say("f32", mat, shape, tiling)
     got = _gpu(gd, shape, gd.F32, None, cb, db, src, T, dft, init=init,
-               variant=gd.PLAIN if tiling is None else gd.TILED, tiling=tiling, mat="dense")
-    _assert_same(got, (wf, (wre, wim)), f"f32 dense {tiling}")
+               variant=gd.PLAIN if tiling is None else gd.TILED, tiling=tiling, mat=mat)
+    _assert_same(got, (wf, (wre, wim)), f"f32 {mat} {tiling}")
 
 
 def test_material_formats(gd, oracle):
@@ -147,16 +148,22 @@ def test_material_formats(gd, oracle):
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=12)
     distinct = np.unique(cb.view(np.uint32)).size
     assert 2 <= distinct <= 256
-    # AUTO picks the indexed form for a sphere medium, with one entry per value
+    # AUTO picks the indexed form for a sphere medium with FP16 fields, with one
+    # entry per value, and the dense form with FP32 fields
     with gd.Context(*shape) as ctx:
+        ctx.set_storage(gd.F16, [0] * 6)
         ctx.set_isotropic(cb, db)
         assert ctx.get_material_format() == (gd.MAT_INDEXED, distinct)
+    with gd.Context(*shape) as ctx:
+        ctx.set_isotropic(cb, db)
+        assert ctx.get_material_format() == (gd.MAT_DENSE, 0)
         with pytest.raises(gd.GdError):
             ctx.set_material_format(gd.MAT_DENSE)  # after the material
     # a continuous medium has too many values: AUTO -> dense, INDEXED -> error
     rng = np.random.default_rng(1)
     cbr = (0.5 / rng.uniform(1.0, 6.0, (nz, ny, nx))).astype(np.float32)
     with gd.Context(*shape) as ctx:
+        ctx.set_storage(gd.F16, [0] * 6)
         ctx.set_isotropic(cbr, db)
         assert ctx.get_material_format() == (gd.MAT_DENSE, 0)
     with gd.Context(*shape) as ctx:
