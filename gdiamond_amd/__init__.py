@@ -26,6 +26,7 @@ COEFF_NAMES = ("Cax", "Cbx", "Cay", "Cby", "Caz", "Cbz",
 PLAIN, TILED = 0, 1
 F32, F64, F16 = 0, 1, 2  # storage formats
 MAT_AUTO, MAT_DENSE, MAT_INDEXED, MAT_FULL = 0, 1, 2, 3  # material formats
+BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PER_INSTANCE = 0, 1, 2  # batch run modes
 
 GD_OK = 0
 ERRORS = {-1: "GD_ERR_ARG", -2: "GD_ERR_STATE", -3: "GD_ERR_HIP", -4: "GD_ERR_NOMEM",
@@ -87,6 +88,27 @@ _SIGS = {
     "gd_set_launch_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gd_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "gd_calibration_copy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _DP]),
+    "gd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_size_t),
+                                  C.c_float]),
+    "gd_batch_destroy": (C.c_int, [C.c_void_p]),
+    "gd_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "gd_batch_set_isotropic": (C.c_int, [C.c_void_p, C.c_int, _FP, C.c_float]),
+    "gd_batch_set_isotropic_eps": (C.c_int, [C.c_void_p, C.c_int, _FP, C.c_float]),
+    "gd_batch_add_source": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_size_t,
+                                      C.c_size_t, _FP, C.c_size_t]),
+    "gd_batch_set_fields": (C.c_int, [C.c_void_p, C.c_int, _FP * 6]),
+    "gd_batch_get_fields": (C.c_int, [C.c_void_p, C.c_int, _FP * 6]),
+    "gd_batch_reset": (C.c_int, [C.c_void_p]),
+    "gd_batch_get_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "gd_batch_num_classes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "gd_batch_class_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_int)]),
+    "gd_batch_set_class_tiling": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tiling)]),
+    "gd_batch_get_class_tiling": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tiling)]),
+    "gd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int]),
+    "gd_batch_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int]),
+    "gd_batch_last_run_stats": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_size_t)]),
+    "gd_batch_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "gd_default_problem_coefficients": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, _FP * 12]),
     "gd_default_mz_table": (C.c_int, [C.c_size_t, _FP]),
     "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
@@ -369,3 +391,115 @@ class Context:
         ms = C.c_double()
         self._ok(lib().gd_calibration_copy(self._h, n, reps, C.byref(ms)))
         return ms.value
+
+
+# ---------------------------------------------------------------- batches
+
+
+class Batch:
+    """Many independent FP32 problems advanced together (a ``gd_batch``)."""
+
+    def __init__(self, dims, dx=1.0):
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        arr = (C.c_size_t * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        h = C.c_void_p()
+        rc = lib().gd_batch_create(C.byref(h), len(self.dims), arr, float(dx))
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_batch_last_error(None).decode())
+        self._h = h
+
+    def _ok(self, rc):
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.dims)
+
+    def _n(self, i):
+        nx, ny, nz = self.dims[i]
+        return nx * ny * nz
+
+    def zyx(self, i):
+        nx, ny, nz = self.dims[i]
+        return (nz, ny, nx)
+
+    def set_isotropic(self, i, cb, db):
+        a = _f32(cb, self._n(i))
+        self._ok(lib().gd_batch_set_isotropic(self._h, int(i), _fp(a), float(db)))
+
+    def set_isotropic_eps(self, i, eps_r, dt):
+        a = _f32(eps_r, self._n(i))
+        self._ok(lib().gd_batch_set_isotropic_eps(self._h, int(i), _fp(a), float(dt)))
+
+    def add_source(self, i, comp, si, sj, sk, table):
+        t = _f32(table)
+        self._ok(lib().gd_batch_add_source(self._h, int(i), int(comp), si, sj, sk, _fp(t), t.size))
+
+    def set_fields(self, i, fields):
+        arrs = [_f32(f, self._n(i)) for f in fields]
+        self._ok(lib().gd_batch_set_fields(self._h, int(i), (_FP * 6)(*[_fp(a) for a in arrs])))
+
+    def get_fields(self, i):
+        out = [np.empty(self.zyx(i), np.float32) for _ in range(6)]
+        self._ok(lib().gd_batch_get_fields(self._h, int(i), (_FP * 6)(*[_fp(a) for a in out])))
+        return out
+
+    def reset(self):
+        self._ok(lib().gd_batch_reset(self._h))
+
+    @property
+    def step(self):
+        s = C.c_size_t()
+        self._ok(lib().gd_batch_get_step(self._h, C.byref(s)))
+        return s.value
+
+    def classes(self):
+        n = C.c_int()
+        self._ok(lib().gd_batch_num_classes(self._h, C.byref(n)))
+        out = []
+        for c in range(n.value):
+            d = (C.c_size_t * 3)()
+            k = C.c_int()
+            self._ok(lib().gd_batch_class_info(self._h, c, d, C.byref(k)))
+            t = Tiling()
+            self._ok(lib().gd_batch_get_class_tiling(self._h, c, C.byref(t)))
+            out.append({"dims": tuple(d), "ninst": k.value, "tiling": t.as_dict()})
+        return out
+
+    def set_class_tiling(self, cls, tiling):
+        t = Tiling(**tiling) if isinstance(tiling, dict) else tiling
+        self._ok(lib().gd_batch_set_class_tiling(self._h, int(cls), C.byref(t)))
+
+    def autotune(self, reps=2):
+        self._ok(lib().gd_batch_autotune(self._h, int(reps)))
+        return self.classes()
+
+    def run(self, nsteps, mode=BATCH_GRAPH):
+        self._ok(lib().gd_batch_run(self._h, int(nsteps), int(mode)))
+
+    def last_run_stats(self):
+        ms, nl = C.c_double(), C.c_size_t()
+        self._ok(lib().gd_batch_last_run_stats(self._h, C.byref(ms), C.byref(nl)))
+        return {"total_ms": ms.value, "nlaunch": nl.value}
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        self._ok(lib().gd_batch_device_bytes(self._h, C.byref(b)))
+        return b.value

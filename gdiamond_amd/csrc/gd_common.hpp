@@ -203,6 +203,57 @@ hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, cons
 struct ZTiling {
   int blt, rows, waves, zchunk;
 };
+
+// Arguments of one TILED launch over one problem: source and destination
+// field copies, decomposition into ntx*nty*ntz tiles of zc planes.
+struct ZArgs {
+  Fields src;
+  Fields dst;
+  Scales sc;
+  Material m;
+  Grid g;
+  Sources s;
+  Dft d;
+  int n0;         // absolute step of level 1
+  int run_step0;  // DFT weight column of level 1
+  int ntx, nty, ntz;
+  int zc;         // planes owned per workgroup
+};
+
+// Tile decomposition of an nx*ny*nz problem under a tiling.
+inline void ztile_counts(const ZTiling& t, int nx, int ny, int nz, int& ntx, int& nty, int& ntz) {
+  const int ox = 64 - 2 * t.blt;
+  const int oy = t.rows * t.waves - 2 * t.blt;
+  ntx = (nx + ox - 1) / ox;
+  nty = (ny + oy - 1) / oy;
+  ntz = (nz + t.zchunk - 1) / t.zchunk;
+}
+
+// Batched persistent launch: every workgroup takes (instance, tile) work items
+// from a queue until it is empty. Instance descriptors come in pairs (field
+// copy parity 0 and 1); the absolute step and each group's parity live in
+// device memory so that a captured graph can be replayed unchanged.
+struct BatchItem {
+  int inst;
+  int tile;
+};
+struct BatchLaunch {
+  const ZArgs* inst;     // [2 * ninst]: src = copy p, dst = copy 1-p for parity p
+  const BatchItem* items;
+  int nitems;
+  unsigned* counter;     // queue position of this launch; zero on entry
+  const int* step;       // absolute step at the start of the run
+  const int* parity;     // copy parity of the group at the start of the run
+  int step_offset;       // steps of the run before this launch
+  int launch_index;      // launches of the group before this one in the run
+};
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
+                                hipStream_t stream);
+// Workgroups of one batch launch that fill the device (occupancy x CUs).
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device);
+// End of a run: step += nsteps, parity[g] ^= flip[g], every queue counter = 0.
+hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* flip, int ngroups,
+                                unsigned* counters, int ncounters, hipStream_t stream);
 bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk);
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,

@@ -43,20 +43,6 @@ namespace gd {
 
 namespace {
 
-struct ZArgs {
-  Fields src;
-  Fields dst;
-  Scales sc;
-  Material m;
-  Grid g;
-  Sources s;
-  Dft d;
-  int n0;         // absolute step of level 1
-  int run_step0;  // DFT weight column of level 1
-  int ntx, nty, ntz;
-  int zc;         // planes owned per workgroup
-};
-
 // Lane l receives the value of lane l-1 (lane 0 receives 0): wave_shr:1.
 __device__ __forceinline__ float from_lane_below(float v) {
   return __builtin_bit_cast(
@@ -119,8 +105,10 @@ constexpr int zstream_waves_per_simd() {
   return R <= 2 ? 4 : 2;
 }
 
+// One workgroup's work: tile `tile` (x fastest, then y, then z chunk) of the
+// problem described by `a`, streamed through B time levels.
 template <int B, int R, int W, int kMat, class S>
-__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(ZArgs a) {
+__device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
   using C = Codec<S>;
   constexpr bool kFull = kMat == kMatFull;
   constexpr bool kIdx = kMat == kMatIdx;
@@ -139,7 +127,6 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % a.ntx;
   const int tyz = tile / a.ntx;
   const int ty = tyz % a.nty;
@@ -465,6 +452,108 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
 }
 
 template <int B, int R, int W, int kMat, class S>
+__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(ZArgs a) {
+  zstream_tile<B, R, W, kMat, S>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Copies a wave-uniform object from global memory into registers through
+// readfirstlane (a scalar value for every word the caller uses).
+template <class T>
+__device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized objects only");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+#pragma unroll
+  for (unsigned q = 0; q < sizeof(T) / 4; ++q) d[q] = __builtin_amdgcn_readfirstlane(s[q]);
+}
+
+// Persistent batch kernel: workgroups take (instance, tile) items from the
+// launch's queue until it runs dry (every wave of a workgroup sees the same
+// item, so all of them leave the loop together). Items of the largest
+// instances come first, so the tail of the launch is made of small tiles.
+template <int B, int R, int W, int kMat, class S>
+__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_batch_kernel(
+    BatchLaunch L) {
+  __shared__ int s_item;
+  const int n0 = *L.step + L.step_offset;
+  const int par = (*L.parity + L.launch_index) & 1;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(L.counter, 1u);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    // every wave has read the item before the next one is taken
+    __syncthreads();
+    if (it >= L.nitems) break;
+    // the item and the instance descriptor are uniform: copy them word by
+    // word through readfirstlane so that they live in SGPRs
+    BatchItem wi;
+    uniform_copy(&wi, L.items + it);
+    ZArgs a;
+    uniform_copy(&a, L.inst + 2 * wi.inst + par);
+    a.n0 = n0;
+    zstream_tile<B, R, W, kMat, S>(a, wi.tile);
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_advance_kernel(int* step, int nsteps, int* parity,
+                                                            const int* flip, int ngroups,
+                                                            unsigned* counters, int ncounters) {
+  const int t = threadIdx.x;
+  if (t == 0) *step += nsteps;
+  for (int g = t; g < ngroups; g += blockDim.x) parity[g] ^= flip[g];
+  for (int c = t; c < ncounters; c += blockDim.x) counters[c] = 0u;
+}
+
+template <int B, int R, int W, int kMat>
+void launch_batch_k(const BatchLaunch& L, int nblocks, hipStream_t stream) {
+  hipLaunchKernelGGL((zstream_batch_kernel<B, R, W, kMat, float>), dim3(nblocks), dim3(64 * W), 0,
+                     stream, L);
+}
+
+template <int B, int R, int W>
+const void* batch_kernel_fn(MatKind mk) {
+  return mk == kMatIdx ? reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIdx, float>)
+                       : reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIso, float>);
+}
+
+template <int R, int W>
+const void* batch_kernel_rw(int blt, MatKind mk) {
+  switch (blt) {
+    case 1: return batch_kernel_fn<1, R, W>(mk);
+    case 2: return batch_kernel_fn<2, R, W>(mk);
+    case 3: return batch_kernel_fn<3, R, W>(mk);
+    case 4: return batch_kernel_fn<4, R, W>(mk);
+    default: return nullptr;
+  }
+}
+
+const void* batch_kernel(const ZTiling& t, MatKind mk) {
+  if (t.rows == 2 && t.waves == 4) return batch_kernel_rw<2, 4>(t.blt, mk);
+  if (t.rows == 2 && t.waves == 8) return batch_kernel_rw<2, 8>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 4) return batch_kernel_rw<4, 4>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 8) return batch_kernel_rw<4, 8>(t.blt, mk);
+  return nullptr;
+}
+
+template <int R, int W>
+hipError_t launch_batch_rw(int blt, MatKind mk, const BatchLaunch& L, int nblocks,
+                           hipStream_t stream) {
+  const bool idx = mk == kMatIdx;
+  switch (blt) {
+    case 1: idx ? launch_batch_k<1, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<1, R, W, kMatIso>(L, nblocks, stream); break;
+    case 2: idx ? launch_batch_k<2, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<2, R, W, kMatIso>(L, nblocks, stream); break;
+    case 3: idx ? launch_batch_k<3, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<3, R, W, kMatIso>(L, nblocks, stream); break;
+    case 4: idx ? launch_batch_k<4, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<4, R, W, kMatIso>(L, nblocks, stream); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int B, int R, int W, int kMat, class S>
 void launch_k(const ZArgs& a, hipStream_t stream) {
   const unsigned blocks = (unsigned)((long long)a.ntx * a.nty * a.ntz);
   hipLaunchKernelGGL((zstream_kernel<B, R, W, kMat, S>), dim3(blocks), dim3(64 * W), 0, stream,
@@ -534,6 +623,34 @@ bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk) {
                   (t.rows == 4 && t.waves == 4) || (t.rows == 4 && t.waves == 8);
   if (!rw) return false;
   return t.rows * t.waves - 2 * t.blt >= 1 && 64 - 2 * t.blt >= 1;
+}
+
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
+                                hipStream_t stream) {
+  if (!ztiling_supported(t, kF32, mk) || mk == kMatFull || nblocks < 1)
+    return hipErrorInvalidValue;
+  if (t.rows == 2 && t.waves == 4) return launch_batch_rw<2, 4>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_batch_rw<2, 8>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_batch_rw<4, 4>(t.blt, mk, L, nblocks, stream);
+  return launch_batch_rw<4, 8>(t.blt, mk, L, nblocks, stream);
+}
+
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device) {
+  const void* fn = batch_kernel(t, mk);
+  if (!fn) return 0;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * t.waves, 0) != hipSuccess)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return 0;
+  return per_cu * cus;
+}
+
+hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* flip, int ngroups,
+                                unsigned* counters, int ncounters, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(256), 0, stream, step, nsteps, parity,
+                     flip, ngroups, counters, ncounters);
+  return hipGetLastError();
 }
 
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,

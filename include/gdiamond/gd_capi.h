@@ -223,6 +223,64 @@ int gd_calibration_copy(gd_ctx* ctx, size_t n, int reps, double* ms);
 /* Number of visible HIP devices (0 when none). */
 int gd_device_count(int* count);
 
+/* ---- batches of independent problems -------------------------------------- */
+
+/* A batch holds many independent FP32 problems of (possibly) different sizes
+ * that advance together. Instances with equal dims form a size class; each
+ * class has a tiling (blt, rows x waves, zchunk) chosen by gd_batch_autotune
+ * or set explicitly. Classes whose tilings share (blt, rows, waves) run in
+ * one persistent launch per blt steps: its workgroups take (instance, tile)
+ * work items from a queue, largest instances first. Launch groups with
+ * different shapes run concurrently as parallel branches.
+ * Per instance the results equal those of a gd_ctx with the same problem run
+ * with GD_VARIANT_TILED (and so the sequential update), bit for bit.
+ * No counterpart in the reference, which runs one problem per process and
+ * fixes its tile shapes with compile-time macros (kernels_mm_ver2.cuh:8-32). */
+typedef struct gd_batch gd_batch;
+
+/* Run modes of gd_batch_run. */
+enum {
+  GD_BATCH_PERSISTENT = 0,  /* persistent launches issued directly */
+  GD_BATCH_GRAPH = 1,       /* the same launches captured once per nsteps in a
+                               hipGraph, replayed on later runs */
+  GD_BATCH_PER_INSTANCE = 2 /* one ordinary TILED launch sequence per instance
+                               (the unbatched way; for comparison) */
+};
+
+/* dims: 3*ninst sizes (nx, ny, nz per instance, each >= 3); every instance has
+ * cell size dx. Fields start at zero. */
+int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx);
+int gd_batch_destroy(gd_batch* b);
+const char* gd_batch_last_error(const gd_batch* b);
+/* Isotropic material of instance `inst` (as gd_set_isotropic / _eps). */
+int gd_batch_set_isotropic(gd_batch* b, int inst, const float* cb, float db);
+int gd_batch_set_isotropic_eps(gd_batch* b, int inst, const float* eps_r, float dt);
+/* Point source of instance `inst` (as gd_add_source). */
+int gd_batch_add_source(gd_batch* b, int inst, int comp, size_t i, size_t j, size_t k,
+                        const float* table, size_t nsteps);
+int gd_batch_set_fields(gd_batch* b, int inst, const float* const fields[6]);
+int gd_batch_get_fields(gd_batch* b, int inst, float* const fields[6]);
+/* Fields of every instance to zero, step counter to 0. */
+int gd_batch_reset(gd_batch* b);
+int gd_batch_get_step(const gd_batch* b, size_t* step);
+/* Size classes (distinct dims, in order of first appearance). */
+int gd_batch_num_classes(const gd_batch* b, int* nclass);
+int gd_batch_class_info(const gd_batch* b, int cls, size_t dims[3], int* ninst);
+int gd_batch_set_class_tiling(gd_batch* b, int cls, const gd_tiling* t);
+int gd_batch_get_class_tiling(const gd_batch* b, int cls, gd_tiling* t);
+/* Autotuner: for each size class, times every supported tiling (blt 1..4,
+ * rows x waves in {2,4}x{4,8}, zchunk in {16,32,64,nz}) as persistent launches
+ * over that class's instances and keeps the fastest per cell-update. Fields
+ * and the step counter are left unchanged (the timing launches always read
+ * the current copy). `reps` timed launches per candidate (>= 1). */
+int gd_batch_autotune(gd_batch* b, int reps);
+/* Advance every instance by nsteps (all source tables must cover them). */
+int gd_batch_run(gd_batch* b, size_t nsteps, int mode);
+/* Wall time of the last gd_batch_run on the device (events around the whole
+ * run) and the number of kernel launches it made. */
+int gd_batch_last_run_stats(const gd_batch* b, double* total_ms, size_t* nlaunch);
+int gd_batch_device_bytes(const gd_batch* b, size_t* bytes);
+
 /* ---- host-side problem builders (no device needed) -------------------------- */
 
 /* The reference default problem: gDiamond::gDiamond coefficients
