@@ -93,6 +93,152 @@ def cpu_baseline(gd, n, eps, dt, src, steps, threads):
     return n ** 3 * steps / dt_s / 1e9
 
 
+ERROR_BOUND = 1e-3  # relative DFT-spectrum error of a form counted as "at fixed error"
+
+
+def spectral_error(re, im, ref_re, ref_im):
+    """max over (component, frequency) of ||X - X_ref||_2 / ||X_ref||_2 on the
+    monitor plane."""
+    X = np.asarray(re, np.float64) + 1j * np.asarray(im, np.float64)
+    R = np.asarray(ref_re, np.float64) + 1j * np.asarray(ref_im, np.float64)
+    return float(np.max(np.linalg.norm(X - R, axis=-1) / np.linalg.norm(R, axis=-1)))
+
+
+def spectra(ctx, gd):
+    out = [ctx.dft_get(c) for c in (gd.EX, gd.EZ)]
+    return np.stack([o[0] for o in out]), np.stack([o[1] for o in out])
+
+
+def error_leg(gd, n, T, eps, dt, src, f_cps, tiled_spec, tiled_ms):
+    """The same workload in other forms against an FP64 PLAIN reference run:
+    DFT-spectrum error and run time of each; the chosen form is the fastest
+    within ERROR_BOUND, compared with the FP32 PLAIN form."""
+    c = n // 2
+
+    def one(storage, variant, scale=None):
+        ctx = gd.Context(n, n, n, dx=1.0)
+        try:
+            if storage != gd.F32:
+                ctx.set_storage(storage, scale)
+            ctx.set_isotropic_eps(eps, dt)
+            ctx.add_source(gd.EZ, c, c, c, src)
+            ctx.dft_config([gd.EX, gd.EZ], axis=2, index=c, f_cps=f_cps)
+            ctx.run(T, variant)
+            ms = ctx.last_run_stats()["total_ms"]
+            return spectra(ctx, gd), ms
+        finally:
+            ctx.close()
+
+    ref, ref_ms = one(gd.F64, gd.PLAIN)
+    print(f"# fp64 plain reference {ref_ms:.0f} ms", file=sys.stderr, flush=True)
+    forms = {"f32-tiled": (tiled_spec, tiled_ms)}
+    forms["f32-plain"] = one(gd.F32, gd.PLAIN)
+    forms["f16-tiled"] = one(gd.F16, gd.TILED, [F16_SCALE_EXP] * 6)
+    rep = {}
+    for name, (spec, ms) in forms.items():
+        rep[name] = {"ms_per_run": round(ms, 2), "GCUPS": round(n ** 3 * T / (ms * 1e-3) / 1e9, 2),
+                     "spectrum_err": float(f"{spectral_error(spec[0], spec[1], *ref):.3e}")}
+        print(f"# {name}: {rep[name]}", file=sys.stderr, flush=True)
+    ok = [k for k in rep if rep[k]["spectrum_err"] <= ERROR_BOUND]
+    chosen = min(ok, key=lambda k: rep[k]["ms_per_run"]) if ok else None
+    return {"bound": ERROR_BOUND, "reference": "fp64 plain, same workload",
+            "error": "max over Ex,Ez and 3 frequencies of ||X-X64||_2/||X64||_2 on z=n/2",
+            "forms": rep, "plain": "f32-plain", "chosen": chosen,
+            "speedup_chosen_vs_plain": (round(rep["f32-plain"]["ms_per_run"] /
+                                              rep[chosen]["ms_per_run"], 3) if chosen else None)}
+
+
+def run_precision(args, gd):
+    """Precision form: FP16 field storage (scale 2^F16_SCALE_EXP, exact FP32 Cb
+    through the indexed material), FP32 arithmetic, FP64 DFT, TILED kernel.
+    Throughput on the 1024^3 sphere medium (seed 43) for 2000 steps; error of
+    the FP16 and FP32 forms against an FP64 run of the identical problem at
+    256^3 (same generator and seed, 2000 steps)."""
+    T = 2000
+    dt = np.float32(0.99 / math.sqrt(3.0))
+    src = gd.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
+    f_cps = np.array([0.03, 0.05, 0.07])
+
+    def make(n, storage, eps):
+        ctx = gd.Context(n, n, n, dx=1.0)
+        if storage == gd.F16:
+            ctx.set_storage(gd.F16, [F16_SCALE_EXP] * 6)
+        elif storage == gd.F64:
+            ctx.set_storage(gd.F64)
+        ctx.set_isotropic_eps(eps, dt)
+        c = n // 2
+        ctx.add_source(gd.EZ, c, c, c, src)
+        ctx.dft_config([gd.EX, gd.EZ], axis=2, index=c, f_cps=f_cps)
+        return ctx
+
+    # error at 256^3
+    ne = 256
+    eps_s = gd.sphere_medium(ne, ne, ne, 43, 200, 4.0, 20.0, 1.0, 12.0)
+    res = {}
+    ref = None
+    for name, storage, variant in (("f64-plain", gd.F64, gd.PLAIN), ("f32-tiled", gd.F32, gd.TILED),
+                                   ("f16-tiled", gd.F16, gd.TILED), ("f16-plain", gd.F16, gd.PLAIN)):
+        ctx = make(ne, storage, eps_s)
+        ctx.run(T, variant)
+        spec = spectra(ctx, gd)
+        ms = ctx.last_run_stats()["total_ms"]
+        ctx.close()
+        if ref is None:
+            ref = spec
+        res[name] = {"ms_per_run": round(ms, 2),
+                     "spectrum_err": float(f"{spectral_error(spec[0], spec[1], *ref):.3e}")}
+        print(f"# 256^3 {name}: {res[name]}", file=sys.stderr, flush=True)
+    del eps_s
+
+    # throughput at 1024^3
+    n = args.size if args.size != 512 else 1024
+    t0 = time.perf_counter()
+    eps = gd.sphere_medium(n, n, n, 43, 200, 4.0, 20.0, 1.0, 12.0)
+    ctx = make(n, gd.F16, eps)
+    del eps
+    fmt, ntable = ctx.get_material_format()
+    ctx.set_launch_timing(True)
+    tiling = ctx.plan_tiling()
+    print(f"# {n}^3 setup {time.perf_counter() - t0:.1f}s tiling={tiling} material={fmt},{ntable}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.reset()
+        ctx.run(T, gd.TILED)
+    ms, kms = [], []
+    for _ in range(args.steps):
+        ctx.reset()
+        ctx.run(T, gd.TILED)
+        st = ctx.last_run_stats()
+        ms.append(st["total_ms"])
+        kms.append(st["kernel_ms"])
+        print(f"# run {st}", file=sys.stderr, flush=True)
+    re, im = ctx.dft_get(gd.EZ)
+    finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
+    ctx.close()
+    cells = float(n) ** 3
+    value = cells * T / (np.mean(ms) * 1e-3) / 1e9
+    b_alg = bytes_alg("f16", "indexed" if fmt == gd.MAT_INDEXED else "dense")
+    achieved = b_alg * cells * T / (np.mean(kms) * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s at 1024^3, FP16 field storage",
+        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(float(np.mean(ms)), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16",
+        "data": "synthetic",
+        "config": {"workload": f"fdtd3d-yee-spheres-{n}^3-{T}steps-f16", "grid": [n, n, n],
+                   "timesteps": T, "medium": "200 spheres seed 43, eps 1-12",
+                   "storage": f"binary16 x 2^{F16_SCALE_EXP}", "tiling": tiling,
+                   "material": "indexed" if fmt == gd.MAT_INDEXED else "dense",
+                   "dft_finite": finite,
+                   "error_256cube": {"reference": "f64-plain", "forms": res,
+                                     "bound": ERROR_BOUND}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "zstream_kernel", "bytes_alg_per_cell_update": b_alg},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args, gd):
     """Batched workload: `batch_size` independent FP32 problems, cube edge drawn
     uniformly from {32,48,64,96,128,160} (numpy default_rng(2024)), each with
@@ -192,8 +338,11 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
-                    help="single: the 512^3 problem; batch: 256 independent problems")
+    ap.add_argument("--no-error-leg", action="store_true",
+                    help="skip the fixed-error comparison (FP64 reference, FP32 PLAIN, FP16)")
+    ap.add_argument("--workload", choices=["single", "batch", "precision"], default="single",
+                    help="single: the 512^3 problem; batch: 256 independent problems; "
+                         "precision: FP16 storage at 1024^3 with its error at 256^3")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--batch-timesteps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -209,6 +358,9 @@ def main():
 
     if args.workload == "batch":
         run_batch(args, gd)
+        return
+    if args.workload == "precision":
+        run_precision(args, gd)
         return
 
     n, T = args.size, args.timesteps
@@ -267,6 +419,8 @@ def main():
     # sanity: the monitor saw the pulse, fields are finite
     re, im = ctx.dft_get(gd.EZ)
     finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
+    run_spec = spectra(ctx, gd)
+    run_ms = ctx.last_run_stats()["total_ms"]
 
     cells = n ** 3
     total = sum(wall)
@@ -310,6 +464,10 @@ def main():
                    "dft_finite": finite},
         "roofline": roofline,
     }
+    if (not args.no_error_leg and args.storage == "f32" and variant == gd.TILED and
+            args.material != "indexed"):
+        ctx.close()
+        out["fixed_error"] = error_leg(gd, n, T, eps, dt, src, f_cps, run_spec, run_ms)
     if not args.no_cpu_baseline and args.storage == "f32":
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(gd, n, eps, dt, src, args.cpu_steps, threads)
