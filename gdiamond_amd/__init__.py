@@ -109,6 +109,17 @@ _SIGS = {
     "gd_batch_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int]),
     "gd_batch_last_run_stats": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_size_t)]),
     "gd_batch_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "gd_nn_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.c_size_t, C.c_size_t]),
+    "gd_nn_destroy": (C.c_int, [C.c_void_p]),
+    "gd_nn_last_error": (C.c_char_p, [C.c_void_p]),
+    "gd_nn_num_params": (C.c_int, [C.POINTER(C.c_size_t)]),
+    "gd_nn_set_params": (C.c_int, [C.c_void_p, _FP, C.c_size_t]),
+    "gd_nn_get_params": (C.c_int, [C.c_void_p, _FP, C.c_size_t]),
+    "gd_nn_set_input": (C.c_int, [C.c_void_p, _FP]),
+    "gd_nn_set_target": (C.c_int, [C.c_void_p, _FP, C.POINTER(C.c_uint8)]),
+    "gd_nn_forward": (C.c_int, [C.c_void_p, _FP]),
+    "gd_nn_train_step": (C.c_int, [C.c_void_p, C.c_float, _FP, _FP]),
+    "gd_nn_mfma_selftest": (C.c_int, [_FP, _FP, _FP]),
     "gd_default_problem_coefficients": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, _FP * 12]),
     "gd_default_mz_table": (C.c_int, [C.c_size_t, _FP]),
     "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
@@ -503,3 +514,89 @@ class Batch:
         b = C.c_size_t()
         self._ok(lib().gd_batch_device_bytes(self._h, C.byref(b)))
         return b.value
+
+
+# ---------------------------------------------------------------- corrector network
+
+
+NN_LAYER_PARAMS = 27 * 16 * 16 + 16
+NN_PARAMS = 3 * NN_LAYER_PARAMS
+NN_CIN = (12, 16, 16)
+NN_COUT = (16, 16, 12)
+
+
+def mfma_selftest(a, b):
+    """D = A B (16x16) through the MFMA operand layout of the network kernels."""
+    A = _f32(a, 256)
+    Bm = _f32(b, 256)
+    D = np.empty(256, np.float32)
+    _check_host(lib().gd_nn_mfma_selftest(_fp(A), _fp(Bm), _fp(D)), "gd_nn_mfma_selftest")
+    return D.reshape(16, 16)
+
+
+class Net:
+    """The corrector network on one grid (a ``gd_nn``): tensors are
+    channels-last arrays shaped (nz, ny, nx, 16)."""
+
+    def __init__(self, nx, ny, nz):
+        self.shape = (int(nx), int(ny), int(nz))
+        self.N = int(nx) * int(ny) * int(nz)
+        h = C.c_void_p()
+        rc = lib().gd_nn_create(C.byref(h), nx, ny, nz)
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_nn_last_error(None).decode())
+        self._h = h
+
+    def _ok(self, rc):
+        if rc != GD_OK:
+            raise GdError(rc, lib().gd_nn_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, p):
+        a = _f32(p, NN_PARAMS)
+        self._ok(lib().gd_nn_set_params(self._h, _fp(a), a.size))
+
+    def get_params(self):
+        out = np.empty(NN_PARAMS, np.float32)
+        self._ok(lib().gd_nn_get_params(self._h, _fp(out), out.size))
+        return out
+
+    def set_input(self, x):
+        a = _f32(x, self.N * 16)
+        self._ok(lib().gd_nn_set_input(self._h, _fp(a)))
+
+    def set_target(self, t, mask):
+        a = _f32(t, self.N * 16)
+        m = np.ascontiguousarray(mask, np.uint8).ravel()
+        if m.size != self.N:
+            raise ValueError("mask must have one byte per cell")
+        self._ok(lib().gd_nn_set_target(self._h, _fp(a), m.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def forward(self):
+        nx, ny, nz = self.shape
+        y = np.empty((nz, ny, nx, 16), np.float32)
+        self._ok(lib().gd_nn_forward(self._h, _fp(y)))
+        return y
+
+    def train_step(self, lr, want_grads=False):
+        loss = C.c_float()
+        g = np.empty(NN_PARAMS, np.float32) if want_grads else None
+        self._ok(lib().gd_nn_train_step(self._h, float(lr), C.byref(loss),
+                                        _fp(g) if g is not None else None))
+        return (loss.value, g) if want_grads else loss.value

@@ -281,6 +281,39 @@ int gd_batch_run(gd_batch* b, size_t nsteps, int mode);
 int gd_batch_last_run_stats(const gd_batch* b, double* total_ms, size_t* nlaunch);
 int gd_batch_device_bytes(const gd_batch* b, size_t* bytes);
 
+/* ---- learned coarse-grid corrector: network kernels --------------------------- */
+
+/* A 3-layer 3x3x3 convolution network (12 -> 16 -> 16 -> 12 channels,
+ * LeakyReLU(1/8) after layers 1 and 2, zero padding) on one nx*ny*nz grid,
+ * BF16 activations and weights, FP32 accumulation (MFMA), FP32 master
+ * parameters with Adam. Parameters: 3 blocks of W[27][16][16] (tap, output
+ * channel, input channel; tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by
+ * b[16], FP32; channels beyond 12 on the input of layer 1 and the output of
+ * layer 3 are padding (held at zero). Tensors are channels-last, 16 FP32
+ * values per cell, cell = x + nx*(y + ny*z). No counterpart in the reference,
+ * which has no learned component. */
+typedef struct gd_nn gd_nn;
+int gd_nn_create(gd_nn** out, size_t nx, size_t ny, size_t nz);
+int gd_nn_destroy(gd_nn* nn);
+const char* gd_nn_last_error(const gd_nn* nn);
+int gd_nn_num_params(size_t* n);
+/* Sets the parameters and resets the Adam moments. */
+int gd_nn_set_params(gd_nn* nn, const float* params, size_t n);
+int gd_nn_get_params(gd_nn* nn, float* params, size_t n);
+/* Input features [cell][16] (rounded to BF16 on upload). */
+int gd_nn_set_input(gd_nn* nn, const float* x);
+/* Training target [cell][16] (channels 0..11 used) and loss mask per cell. */
+int gd_nn_set_target(gd_nn* nn, const float* target, const uint8_t* mask);
+/* Forward pass; y_out (optional) receives the FP32 output [cell][16]. */
+int gd_nn_forward(gd_nn* nn, float* y_out);
+/* Forward, MSE over masked cells x 12 channels, backward, Adam step with
+ * learning rate lr (lr <= 0: gradients only). loss: before the update;
+ * grads_out (optional): the parameter gradients. */
+int gd_nn_train_step(gd_nn* nn, float lr, float* loss, float* grads_out);
+/* D = A B for 16x16 matrices (row-major floats, rounded to BF16) computed
+ * with the MFMA operand layout the convolutions use. */
+int gd_nn_mfma_selftest(const float* a, const float* b, float* d);
+
 /* ---- host-side problem builders (no device needed) -------------------------- */
 
 /* The reference default problem: gDiamond::gDiamond coefficients
