@@ -239,6 +239,73 @@ def run_precision(args, gd):
     print(json.dumps(out), flush=True)
 
 
+def run_learned(args, gd):
+    """Learned form: 64 seeded fine/coarse solver pairs (fine 96^3, coarse
+    48^3, 8 snapshots each) train the corrector; a blend factor alpha for the
+    applied correction is chosen on a held-out problem (fine 128^3 x 400
+    steps); then the fine 512^3 x 2000-step run (eps 1-6, seed 6000) is the
+    judge of the coarse 256^3 x 1000-step run with and without the corrector
+    (every 4 coarse steps): relative DFT-spectrum error of Ex, Ey on the mid
+    plane and run time."""
+    from gdiamond_amd import learned
+    t0 = time.perf_counter()
+    raw = learned.make_samples(npairs=args.learned_pairs, n=96, nsnap=8,
+                               log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+    gen_s = time.perf_counter() - t0
+    dx_over_dt = float(np.float32(2.0) / (np.float32(2.0) * learned.DT_FINE))
+    cfg = learned.fit_scales(raw, dx_over_dt)
+    base = learned.baseline_loss(raw, cfg)
+    t0 = time.perf_counter()
+    params, hist = learned.train(raw, cfg, epochs=args.learned_epochs, lr=1e-3, lr_min=1e-4,
+                                 log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+    train_s = time.perf_counter() - t0
+    del raw
+    out_dir = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(out_dir, exist_ok=True)
+    np.savez(os.path.join(out_dir, "corrector_params.npz"), params=params, cfg=np.asarray(cfg))
+
+    def with_alpha(a):
+        c = list(cfg)
+        c[3] = cfg[3] / a
+        c[4] = cfg[4] / a
+        return tuple(c)
+
+    val = {}
+    for a in (0.125, 0.25, 0.5, 1.0):
+        r = learned.compare(128, 400, params, with_alpha(a), seed=6100, nspheres=3)
+        val[a] = (float(np.max(r["learned_err"])), float(np.max(r["coarse_err"])))
+        print(f"# validation alpha {a}: learned {val[a][0]:.4f} coarse {val[a][1]:.4f}",
+              file=sys.stderr, flush=True)
+    alpha = min(val, key=lambda a: val[a][0])
+    n = args.size
+    T = args.timesteps * 2  # fine steps (2000 at the default 1000)
+    res = learned.compare(n, T, params, with_alpha(alpha), seed=6000, nspheres=200)
+    err_learned = float(np.max(res["learned_err"]))
+    err_coarse = float(np.max(res["coarse_err"]))
+    out = {
+        "metric": "time-to-solution and DFT-spectrum error, learned coarse grid vs fine grid",
+        "value": round(res["fine_ms"] / res["learned_ms"], 3),
+        "unit": "x faster than the fine run",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(res["learned_ms"], 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+        "data": "synthetic",
+        "config": {"workload": f"fdtd3d-yee-learned-{n}^3-{T}steps",
+                   "fine": f"{n}^3 x {T} steps, dx 1", "coarse": f"{n // 2}^3 x {T // 2} steps, dx 2",
+                   "medium": "200 spheres seed 6000, eps 1-6, radius 4-20 fine cells",
+                   "network": "3x3x3 conv 12-16-16-12, BF16 MFMA, FP32 accumulate",
+                   "training": {"pairs": args.learned_pairs, "fine": "96^3", "snapshots": 8,
+                                "epochs": args.learned_epochs, "gen_s": round(gen_s, 1),
+                                "train_s": round(train_s, 1), "loss_first": hist[0],
+                                "loss_last": hist[-1], "zero_correction_loss": base},
+                   "alpha_validation": {str(k): v for k, v in val.items()}, "alpha": alpha,
+                   "fine_ms": round(res["fine_ms"], 1), "coarse_ms": round(res["coarse_ms"], 1),
+                   "learned_ms": round(res["learned_ms"], 1),
+                   "spectrum_err_coarse": err_coarse, "spectrum_err_learned": err_learned,
+                   "errors": {k: res[k] for k in ("coarse_err", "learned_err")}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args, gd):
     """Batched workload: `batch_size` independent FP32 problems, cube edge drawn
     uniformly from {32,48,64,96,128,160} (numpy default_rng(2024)), each with
@@ -340,10 +407,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-leg", action="store_true",
                     help="skip the fixed-error comparison (FP64 reference, FP32 PLAIN, FP16)")
-    ap.add_argument("--workload", choices=["single", "batch", "precision"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "precision", "learned"],
+                    default="single",
                     help="single: the 512^3 problem; batch: 256 independent problems; "
                          "precision: FP16 storage at 1024^3 with its error at 256^3")
     ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--learned-pairs", type=int, default=64)
+    ap.add_argument("--learned-epochs", type=int, default=6)
     ap.add_argument("--batch-timesteps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to calibrate HBM counters")
@@ -361,6 +431,9 @@ def main():
         return
     if args.workload == "precision":
         run_precision(args, gd)
+        return
+    if args.workload == "learned":
+        run_learned(args, gd)
         return
 
     n, T = args.size, args.timesteps

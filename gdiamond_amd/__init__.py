@@ -120,6 +120,9 @@ _SIGS = {
     "gd_nn_forward": (C.c_int, [C.c_void_p, _FP]),
     "gd_nn_train_step": (C.c_int, [C.c_void_p, C.c_float, _FP, _FP]),
     "gd_nn_mfma_selftest": (C.c_int, [_FP, _FP, _FP]),
+    "gd_restrict_fields": (C.c_int, [C.c_void_p, _FP * 6]),
+    "gd_run_corrected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP,
+                                   C.POINTER(Tiling)]),
     "gd_default_problem_coefficients": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, _FP * 12]),
     "gd_default_mz_table": (C.c_int, [C.c_size_t, _FP]),
     "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
@@ -397,6 +400,24 @@ class Context:
         b = C.c_size_t()
         self._ok(lib().gd_device_bytes(self._h, C.byref(b)))
         return b.value
+
+    def restrict_fields(self):
+        """Fields restricted to the grid of twice the cell size (Yee-consistent
+        edge / face means), 6 arrays shaped (nz//2, ny//2, nx//2)."""
+        nx, ny, nz = self.shape
+        out = [np.empty((nz // 2, ny // 2, nx // 2), np.float32) for _ in range(6)]
+        self._ok(lib().gd_restrict_fields(self._h, (_FP * 6)(*[_fp(a) for a in out])))
+        return out
+
+    def run_corrected(self, net, nsteps, interval, cfg, tiling=None):
+        """TILED run with the corrector `net` applied every `interval` steps;
+        cfg = (se, sh, sd, re, rh, dx/dt)."""
+        c = _f32(cfg, 6)
+        t = None
+        if tiling is not None:
+            t = Tiling(**tiling) if isinstance(tiling, dict) else tiling
+        self._ok(lib().gd_run_corrected(self._h, net._h, int(nsteps), int(interval), _fp(c),
+                                        C.byref(t) if t is not None else None))
 
     def calibration_copy(self, n, reps=10):
         ms = C.c_double()

@@ -73,6 +73,8 @@ struct gd_ctx {
   double last_total_ms = 0.0, last_kernel_ms = 0.0;
   size_t last_nlaunch = 0;
 
+  float* prev_e[3] = {nullptr, nullptr, nullptr};  // E one step back (learned corrector)
+  float* restrict_buf = nullptr;
   float* cal[2] = {nullptr, nullptr};
   size_t cal_n = 0;
 
@@ -314,6 +316,9 @@ int gd_destroy(gd_ctx* c) {
   for (int k = 0; k < 2; ++k)
     for (int a = 0; a < 6; ++a)
       if (c->f[k][a]) (void)hipFree(c->f[k][a]);
+  for (auto& p : c->prev_e)
+    if (p) (void)hipFree(p);
+  if (c->restrict_buf) (void)hipFree(c->restrict_buf);
   free_material(c);
   for (auto& s : c->src) (void)hipFree(s.table);
   if (c->src_desc) (void)hipFree(c->src_desc);
@@ -925,6 +930,67 @@ int gd_calibration_copy(gd_ctx* c, size_t n, int reps, double* ms) {
   float t = 0.0f;
   GD_HIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
   if (ms) *ms = t / reps;
+  return GD_OK;
+}
+
+// ---- learned coarse-grid corrector -------------------------------------------
+
+int gd_restrict_fields(gd_ctx* c, float* const out[6]) {
+  if (!c) return GD_ERR_ARG;
+  if (!out) return c->fail(GD_ERR_ARG, "null output table");
+  for (int a = 0; a < 6; ++a)
+    if (!out[a]) return c->fail(GD_ERR_ARG, "null output array");
+  if (c->storage != GD_STORAGE_F32) return c->fail(GD_ERR_UNSUPPORTED, "FP32 storage only");
+  const size_t cn = (c->nx / 2) * (c->ny / 2) * (c->nz / 2);
+  if (cn == 0) return c->fail(GD_ERR_ARG, "grid too small to restrict");
+  if (!c->restrict_buf) {
+    hipError_t e = hipMalloc(&c->restrict_buf, 6 * cn * sizeof(float));
+    if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(restriction)");
+  }
+  GD_HIP(c, gd::launch_restrict(fields_of(c, c->cur), (int)c->nx, (int)c->ny, (int)c->nz,
+                                c->restrict_buf, c->stream));
+  GD_HIP(c, hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 6; ++a)
+    GD_HIP(c, hipMemcpy(out[a], c->restrict_buf + a * cn, cn * sizeof(float),
+                        hipMemcpyDeviceToHost));
+  return GD_OK;
+}
+
+int gd_run_corrected(gd_ctx* c, gd_nn* nn, size_t nsteps, int interval, const float cfg[6],
+                     const gd_tiling* tiling) {
+  if (!c) return GD_ERR_ARG;
+  if (!nn || !cfg) return c->fail(GD_ERR_ARG, "null network or configuration");
+  if (interval < 2) return c->fail(GD_ERR_ARG, "interval must be >= 2");
+  if (c->storage != GD_STORAGE_F32 || !c->have_mat || c->mat_kind != gd::kMatIso)
+    return c->fail(GD_ERR_UNSUPPORTED, "the corrector needs FP32 fields and the dense isotropic "
+                                       "material");
+  int nx, ny, nz;
+  gd::nn_dims(nn, &nx, &ny, &nz);
+  if ((size_t)nx != c->nx || (size_t)ny != c->ny || (size_t)nz != c->nz)
+    return c->fail(GD_ERR_ARG, "network grid differs from the context grid");
+  for (int q = 0; q < 5; ++q)
+    if (!(cfg[q] > 0.0f)) return c->fail(GD_ERR_ARG, "scales must be positive");
+  for (auto& p : c->prev_e)
+    if (!p) {
+      hipError_t e = hipMalloc(&p, c->N * sizeof(float));
+      if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(previous E)");
+    }
+  gd::NnFeat nf{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5]};
+  size_t done = 0;
+  while (done + (size_t)interval <= nsteps) {
+    int rc = gd_run(c, GD_VARIANT_TILED, (size_t)interval - 1, tiling);
+    if (rc != GD_OK) return rc;
+    for (int a = 0; a < 3; ++a)
+      GD_HIP(c, hipMemcpyAsync(c->prev_e[a], c->f[c->cur][a], c->N * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+    rc = gd_run(c, GD_VARIANT_TILED, 1, tiling);
+    if (rc != GD_OK) return rc;
+    GD_HIP(c, gd::nn_correct(nn, fields_of(c, c->cur), c->prev_e[0], c->prev_e[1], c->prev_e[2],
+                             static_cast<const float*>(c->cb), nf, c->stream));
+    GD_HIP(c, hipStreamSynchronize(c->stream));
+    done += (size_t)interval;
+  }
+  if (done < nsteps) return gd_run(c, GD_VARIANT_TILED, nsteps - done, tiling);
   return GD_OK;
 }
 

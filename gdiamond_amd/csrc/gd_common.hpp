@@ -194,6 +194,7 @@ __device__ __forceinline__ void source_terms(const Sources& s, int base_comp, in
 
 }  // namespace gd
 
+struct gd_nn;
 // Host-side launchers (defined in the .hip kernel files).
 namespace gd {
 hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, const Material& m,
@@ -261,6 +262,19 @@ hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const
                           hipStream_t stream);
 
 hipError_t launch_copy(const float* src, float* dst, size_t n, hipStream_t stream);
+
+// Learned corrector (gd_learned.hip). Feature and output scales of the
+// network: features E*se, H*sh, (E-E_prev)*sd; corrections y/re, y/rh.
+struct NnFeat {
+  float se, sh, sd;
+  float re, rh;
+  float dx_over_dt;  // 1/eps = cb * dx/dt
+};
+hipError_t launch_restrict(const Fields& f, int nx, int ny, int nz, float* out, hipStream_t s);
+int nn_dims(const ::gd_nn* o, int* nx, int* ny, int* nz);
+// features -> network -> corrected fields, all on stream s
+hipError_t nn_correct(::gd_nn* o, const Fields& f, const float* pe0, const float* pe1,
+                      const float* pe2, const float* cb, const NnFeat& nf, hipStream_t s);
 
 // Accumulates the DFT of `nlev` consecutive steps from the per-level plane
 // snapshot the TILED kernel wrote (run steps run_step0 .. run_step0+nlev-1).

@@ -243,6 +243,99 @@ __global__ __launch_bounds__(256) void nn_adam_kernel(float* __restrict__ p,
   p[q] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
 }
 
+// Input features of the corrector at one cell (channels-last, 16 values):
+//   0-2  E * se            3-5  H * sh           6-8  (E - E_prev) * sd
+//   9    (eps - 1) / 5     10   1/eps - 1        11   1 when a face neighbour
+//                                                     has another eps, else 0
+//   12-15 zero
+// with 1/eps = cb * dx/dt of the isotropic material.
+__global__ __launch_bounds__(256) void nn_features_kernel(Fields f, const float* __restrict__ pe0,
+                                                          const float* __restrict__ pe1,
+                                                          const float* __restrict__ pe2,
+                                                          const float* __restrict__ cb,
+                                                          NnFeat nf, int nx, int ny, int nz,
+                                                          uint16_t* __restrict__ x) {
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long sxy = (long long)nx * ny;
+  if (c >= sxy * nz) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sxy);
+  const float* F[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) F[q] = static_cast<const float*>(f.c[q]);
+  float v[16];
+  v[0] = F[0][c] * nf.se;
+  v[1] = F[1][c] * nf.se;
+  v[2] = F[2][c] * nf.se;
+  v[3] = F[3][c] * nf.sh;
+  v[4] = F[4][c] * nf.sh;
+  v[5] = F[5][c] * nf.sh;
+  v[6] = (F[0][c] - pe0[c]) * nf.sd;
+  v[7] = (F[1][c] - pe1[c]) * nf.sd;
+  v[8] = (F[2][c] - pe2[c]) * nf.sd;
+  const float cc = cb[c];
+  const float inv_eps = cc * nf.dx_over_dt;
+  v[9] = (1.0f / inv_eps - 1.0f) * 0.2f;
+  v[10] = inv_eps - 1.0f;
+  bool edge = false;
+  if (i > 0) edge |= cb[c - 1] != cc;
+  if (i + 1 < nx) edge |= cb[c + 1] != cc;
+  if (j > 0) edge |= cb[c - nx] != cc;
+  if (j + 1 < ny) edge |= cb[c + nx] != cc;
+  if (k > 0) edge |= cb[c - sxy] != cc;
+  if (k + 1 < nz) edge |= cb[c + sxy] != cc;
+  v[11] = edge ? 1.0f : 0.0f;
+#pragma unroll
+  for (int q = 12; q < 16; ++q) v[q] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 16; q += 4) {
+    bf16x4 r;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) r[t] = (short)f32_to_bf16(v[q + t]);
+    *reinterpret_cast<bf16x4*>(x + c * 16 + q) = r;
+  }
+}
+
+// Correction of the fields from the network output: F_q += y[cell][q] / r_q
+// (r = re for E, rh for H) on interior cells only (the walls stay frozen).
+__global__ __launch_bounds__(256) void nn_apply_kernel(Fields f, const float* __restrict__ y,
+                                                       NnFeat nf, int nx, int ny, int nz) {
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long sxy = (long long)nx * ny;
+  if (c >= sxy * nz) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sxy);
+  if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) return;
+  const float ie = 1.0f / nf.re, ih = 1.0f / nf.rh;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    float* F = static_cast<float*>(f.c[q]);
+    F[c] = F[c] + y[c * 16 + q] * (q < 3 ? ie : ih);
+  }
+}
+
+// Yee-consistent restriction of fine fields (cell size h) to a grid of
+// cell size 2h with cnx = nx/2, ... cells: E components average the two fine
+// edges along their axis, H components the four fine faces of their face.
+__global__ __launch_bounds__(256) void restrict_kernel(Fields f, int nx, int ny, int nz, int cnx,
+                                                       int cny, int cnz, float* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long csxy = (long long)cnx * cny;
+  const long long cn = csxy * cnz;
+  if (c >= cn) return;
+  const int I = (int)(c % cnx), J = (int)((c / cnx) % cny), K = (int)(c / csxy);
+  const long long sxy = (long long)nx * ny;
+  const long long o = 2ll * I + 2ll * J * nx + 2ll * K * sxy;
+  const long long dxo = 1, dyo = nx, dzo = sxy;
+  const float* F[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) F[q] = static_cast<const float*>(f.c[q]);
+  out[0 * cn + c] = 0.5f * (F[0][o] + F[0][o + dxo]);
+  out[1 * cn + c] = 0.5f * (F[1][o] + F[1][o + dyo]);
+  out[2 * cn + c] = 0.5f * (F[2][o] + F[2][o + dzo]);
+  out[3 * cn + c] = 0.25f * ((F[3][o] + F[3][o + dyo]) + (F[3][o + dzo] + F[3][o + dyo + dzo]));
+  out[4 * cn + c] = 0.25f * ((F[4][o] + F[4][o + dxo]) + (F[4][o + dzo] + F[4][o + dxo + dzo]));
+  out[5 * cn + c] = 0.25f * ((F[5][o] + F[5][o + dxo]) + (F[5][o + dyo] + F[5][o + dxo + dyo]));
+}
+
 // Parameter block of one layer: W[27][16][16] then b[16] (FP32, 6928 floats).
 constexpr int kLayerParams = 27 * 256 + 16;
 
@@ -339,21 +432,23 @@ int pack_weights(gd_nn* o) {
   return GD_OK;
 }
 
-int forward(gd_nn* o) {
+int forward_on(gd_nn* o, hipStream_t s) {
   const gd::NnGrid g = nn_grid(o);
   const unsigned nb = conv_blocks(o);
   const size_t L = gd::kLayerParams;
-  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, o->stream, o->x, o->wf,
+  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->x, o->wf,
                      o->p + 27 * 256, (void*)o->a1, (const uint16_t*)nullptr, g);
-  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, o->stream, o->a1,
+  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->a1,
                      o->wf + 27 * 256, o->p + L + 27 * 256, (void*)o->a2, (const uint16_t*)nullptr,
                      g);
-  hipLaunchKernelGGL((gd::conv3_kernel<0, 1>), dim3(nb), dim3(256), 0, o->stream, o->a2,
+  hipLaunchKernelGGL((gd::conv3_kernel<0, 1>), dim3(nb), dim3(256), 0, s, o->a2,
                      o->wf + 2 * 27 * 256, o->p + 2 * L + 27 * 256, (void*)o->y,
                      (const uint16_t*)nullptr, g);
   NH(o, hipGetLastError());
   return GD_OK;
 }
+
+int forward(gd_nn* o) { return forward_on(o, o->stream); }
 
 int wgrad(gd_nn* o, const uint16_t* dz, const uint16_t* act, float* gout) {
   const gd::NnGrid g = nn_grid(o);
@@ -595,3 +690,37 @@ extern "C" int gd_nn_mfma_selftest(const float* a, const float* b, float* d) {
   (void)hipFree(dd);
   return e == hipSuccess ? GD_OK : GD_ERR_HIP;
 }
+
+namespace gd {
+
+hipError_t launch_restrict(const Fields& f, int nx, int ny, int nz, float* out, hipStream_t s) {
+  const int cnx = nx / 2, cny = ny / 2, cnz = nz / 2;
+  const long long cn = (long long)cnx * cny * cnz;
+  if (cn == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, s, f, nx,
+                     ny, nz, cnx, cny, cnz, out);
+  return hipGetLastError();
+}
+
+int nn_dims(const gd_nn* o, int* nx, int* ny, int* nz) {
+  *nx = o->nx;
+  *ny = o->ny;
+  *nz = o->nz;
+  return GD_OK;
+}
+
+hipError_t nn_correct(gd_nn* o, const Fields& f, const float* pe0, const float* pe1,
+                      const float* pe2, const float* cb, const NnFeat& nf, hipStream_t s) {
+  const long long n = o->ncell;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(nn_features_kernel, dim3(nb), dim3(256), 0, s, f, pe0, pe1, pe2, cb, nf, o->nx,
+                     o->ny, o->nz, o->x);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (forward_on(o, s) != GD_OK) return hipErrorLaunchFailure;
+  hipLaunchKernelGGL(nn_apply_kernel, dim3(nb), dim3(256), 0, s, f, (const float*)o->y, nf, o->nx,
+                     o->ny, o->nz);
+  return hipGetLastError();
+}
+
+}  // namespace gd

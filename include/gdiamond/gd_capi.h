@@ -310,6 +310,22 @@ int gd_nn_forward(gd_nn* nn, float* y_out);
  * learning rate lr (lr <= 0: gradients only). loss: before the update;
  * grads_out (optional): the parameter gradients. */
 int gd_nn_train_step(gd_nn* nn, float lr, float* loss, float* grads_out);
+/* Restriction of the current fields of a context (cell size h, FP32) to the
+ * grid of cell size 2h with nx/2 x ny/2 x nz/2 cells, consistent with the Yee
+ * positions (Ex(i+1/2,j,k), Hx(i,j+1/2,k+1/2), ...): each coarse E component
+ * is the mean of the two fine edges along its axis, each H component the mean
+ * of the four fine faces of its face. out: 6 host arrays of the coarse size. */
+int gd_restrict_fields(gd_ctx* ctx, float* const out[6]);
+/* GD_VARIANT_TILED run with the corrector applied after every `interval`
+ * steps (the last step of each interval runs on its own so that E one step
+ * back is known): features from E, H, E - E_prev and the material, network
+ * forward, F += y / r on interior cells. cfg = {se, sh, sd, re, rh, dx/dt}:
+ * feature scales of E, H, E - E_prev, output scales of E and H, and dx/dt of
+ * the problem (1/eps = cb*dx/dt). Steps past the last full interval run
+ * uncorrected. FP32 fields, dense isotropic material, network on the same
+ * grid. */
+int gd_run_corrected(gd_ctx* ctx, gd_nn* nn, size_t nsteps, int interval, const float cfg[6],
+                     const gd_tiling* tiling);
 /* D = A B for 16x16 matrices (row-major floats, rounded to BF16) computed
  * with the MFMA operand layout the convolutions use. */
 int gd_nn_mfma_selftest(const float* a, const float* b, float* d);
