@@ -65,53 +65,77 @@ struct NnGrid {
 //         kOut == 1: fp32 [cell][16] after bias (no activation)
 //   kAct: 0 none, 1 LeakyReLU (forward), 2 multiply by LeakyReLU'(mask[cell][m])
 //         (backward: mask holds the post-activation values of that layer)
+// The input is staged in LDS: a workgroup of 4 waves
+// owns a 16 x 4 x 4 output tile; its input with a one-cell halo
+// (18 x 6 x 6 cells x 16 channels, 20.7 KB) is loaded once, every wave keeps
+// the 27 weight operands of its lanes in registers and computes 4 rows of 16
+// cells (27 MFMA each) from LDS.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+
 template <int kAct, int kOut>
-__global__ __launch_bounds__(256) void conv3_kernel(const uint16_t* __restrict__ in,
-                                                    const uint16_t* __restrict__ w,
-                                                    const float* __restrict__ bias,
-                                                    void* __restrict__ out,
-                                                    const uint16_t* __restrict__ mask, NnGrid g) {
-  const int lane = threadIdx.x & 63;
-  const long long gw = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const long long nwaves = (long long)g.nsegx * g.ny * g.nz;
-  if (gw >= nwaves) return;  // whole wave
-  const int seg = (int)(gw % g.nsegx);
-  const int y = (int)((gw / g.nsegx) % g.ny);
-  const int z = (int)(gw / ((long long)g.nsegx * g.ny));
-  const int n = lane & 15, kq = lane >> 4;
-  const int x = seg * 16 + n;
+__global__ __launch_bounds__(256) void conv3_lds_kernel(const uint16_t* __restrict__ in,
+                                                        const uint16_t* __restrict__ w,
+                                                        const float* __restrict__ bias,
+                                                        void* __restrict__ out,
+                                                        const uint16_t* __restrict__ mask,
+                                                        NnGrid g) {
+  __shared__ bf16x4 tile[kHZ * kHY * kHX * 4];
+  const int ntx = (g.nx + kTX - 1) / kTX, nty = (g.ny + kTY - 1) / kTY;
+  const int bx = blockIdx.x % ntx;
+  const int by = (blockIdx.x / ntx) % nty;
+  const int bz = blockIdx.x / (ntx * nty);
+  const int x0 = bx * kTX, y0 = by * kTY, z0 = bz * kTZ;
   const long long sxy = (long long)g.nx * g.ny;
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int tap = 0; tap < 27; ++tap) {
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const int xx = x + dx, yy = y + dy, zz = z + dz;
-    const bool ok = xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
-    bf16x4 b = {0, 0, 0, 0};
-    if (ok) {
-      const long long c = (long long)xx + (long long)yy * g.nx + (long long)zz * sxy;
-      b = *reinterpret_cast<const bf16x4*>(in + c * 16 + 4 * kq);
+  for (int q = threadIdx.x; q < kHZ * kHY * kHX * 4; q += 256) {
+    const int cq = q >> 2, ch = q & 3;
+    const int hx = cq % kHX, hy = (cq / kHX) % kHY, hz = cq / (kHX * kHY);
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+    bf16x4 v = {0, 0, 0, 0};
+    if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
+      v = *reinterpret_cast<const bf16x4*>(
+          in + ((long long)gx + (long long)gy * g.nx + (long long)gz * sxy) * 16 + 4 * ch);
+    tile[q] = v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  bf16x4 wr[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t)
+    wr[t] = *reinterpret_cast<const bf16x4*>(w + ((long long)t * 16 + n) * 16 + 4 * kq);
+  __syncthreads();
+  const int x = x0 + n;
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    const int row = wv * 4 + rr;  // 0..15
+    const int ty = row % kTY, tz = row / kTY;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const int hc = ((tz + 1 + dz) * kHY + (ty + 1 + dy)) * kHX + (n + 1 + dx);
+      acc = mfma16(wr[t], tile[hc * 4 + kq], acc);
     }
-    const bf16x4 a = *reinterpret_cast<const bf16x4*>(w + ((long long)tap * 16 + n) * 16 + 4 * kq);
-    acc = mfma16(a, b, acc);
-  }
-  if (x >= g.nx) return;
-  const long long c = (long long)x + (long long)y * g.nx + (long long)z * sxy;
-  float v[4];
+    const int y = y0 + ty, z = z0 + tz;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) continue;
+    const long long c = (long long)x + (long long)y * g.nx + (long long)z * sxy;
+    float v[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int m = 4 * kq + j;
-    v[j] = acc[j] + (bias ? bias[m] : 0.0f);
-    if (kAct == 1) v[j] = v[j] > 0.0f ? v[j] : v[j] * kLeak;
-    if (kAct == 2) v[j] = bf16_to_f32(mask[c * 16 + m]) > 0.0f ? v[j] : v[j] * kLeak;
-  }
-  if (kOut == 1) {
-    float* o = static_cast<float*>(out) + c * 16 + 4 * kq;
-    *reinterpret_cast<float4*>(o) = make_float4(v[0], v[1], v[2], v[3]);
-  } else {
-    bf16x4 r;
+    for (int j = 0; j < 4; ++j) {
+      const int m = 4 * kq + j;
+      v[j] = acc[j] + (bias ? bias[m] : 0.0f);
+      if (kAct == 1) v[j] = v[j] > 0.0f ? v[j] : v[j] * kLeak;
+      if (kAct == 2) v[j] = bf16_to_f32(mask[c * 16 + m]) > 0.0f ? v[j] : v[j] * kLeak;
+    }
+    if (kOut == 1) {
+      float* o = static_cast<float*>(out) + c * 16 + 4 * kq;
+      *reinterpret_cast<float4*>(o) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      bf16x4 r;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) r[j] = (short)f32_to_bf16(v[j]);
-    *reinterpret_cast<bf16x4*>(static_cast<uint16_t*>(out) + c * 16 + 4 * kq) = r;
+      for (int j = 0; j < 4; ++j) r[j] = (short)f32_to_bf16(v[j]);
+      *reinterpret_cast<bf16x4*>(static_cast<uint16_t*>(out) + c * 16 + 4 * kq) = r;
+    }
   }
 }
 
@@ -419,8 +443,9 @@ gd::NnGrid nn_grid(const gd_nn* o) {
 }
 
 unsigned conv_blocks(const gd_nn* o) {
-  const long long waves = (long long)((o->nx + 15) / 16) * o->ny * o->nz;
-  return (unsigned)((waves + 3) / 4);
+  const long long t = (long long)((o->nx + gd::kTX - 1) / gd::kTX) *
+                      ((o->ny + gd::kTY - 1) / gd::kTY) * ((o->nz + gd::kTZ - 1) / gd::kTZ);
+  return (unsigned)t;
 }
 
 int pack_weights(gd_nn* o) {
@@ -436,12 +461,12 @@ int forward_on(gd_nn* o, hipStream_t s) {
   const gd::NnGrid g = nn_grid(o);
   const unsigned nb = conv_blocks(o);
   const size_t L = gd::kLayerParams;
-  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->x, o->wf,
+  hipLaunchKernelGGL((gd::conv3_lds_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->x, o->wf,
                      o->p + 27 * 256, (void*)o->a1, (const uint16_t*)nullptr, g);
-  hipLaunchKernelGGL((gd::conv3_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->a1,
+  hipLaunchKernelGGL((gd::conv3_lds_kernel<1, 0>), dim3(nb), dim3(256), 0, s, o->a1,
                      o->wf + 27 * 256, o->p + L + 27 * 256, (void*)o->a2, (const uint16_t*)nullptr,
                      g);
-  hipLaunchKernelGGL((gd::conv3_kernel<0, 1>), dim3(nb), dim3(256), 0, s, o->a2,
+  hipLaunchKernelGGL((gd::conv3_lds_kernel<0, 1>), dim3(nb), dim3(256), 0, s, o->a2,
                      o->wf + 2 * 27 * 256, o->p + 2 * L + 27 * 256, (void*)o->y,
                      (const uint16_t*)nullptr, g);
   NH(o, hipGetLastError());
@@ -608,13 +633,13 @@ int gd_nn_train_step(gd_nn* o, float lr, float* loss, float* grads_out) {
   // layer 3
   rc = wgrad(o, o->dz3, o->a2, o->g + 2 * L);
   if (rc != GD_OK) return rc;
-  hipLaunchKernelGGL((gd::conv3_kernel<2, 0>), dim3(nb), dim3(256), 0, o->stream,
+  hipLaunchKernelGGL((gd::conv3_lds_kernel<2, 0>), dim3(nb), dim3(256), 0, o->stream,
                      (const uint16_t*)o->dz3, (const uint16_t*)(o->wb + 2 * 27 * 256),
                      (const float*)nullptr, (void*)o->dz2, (const uint16_t*)o->a2, g);
   // layer 2
   rc = wgrad(o, o->dz2, o->a1, o->g + L);
   if (rc != GD_OK) return rc;
-  hipLaunchKernelGGL((gd::conv3_kernel<2, 0>), dim3(nb), dim3(256), 0, o->stream,
+  hipLaunchKernelGGL((gd::conv3_lds_kernel<2, 0>), dim3(nb), dim3(256), 0, o->stream,
                      (const uint16_t*)o->dz2, (const uint16_t*)(o->wb + 27 * 256),
                      (const float*)nullptr, (void*)o->dz1, (const uint16_t*)o->a1, g);
   // layer 1
