@@ -123,6 +123,10 @@ _SIGS = {
     "gd_restrict_fields": (C.c_int, [C.c_void_p, _FP * 6]),
     "gd_run_corrected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP,
                                    C.POINTER(Tiling)]),
+    "gd_field_rgb": (C.c_int, [_FP, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_uint8)]),
+    "gd_save_field_png": (C.c_int, [_FP, C.c_char_p, C.c_int, C.c_int, C.c_float]),
+    "gd_save_mask_png": (C.c_int, [C.POINTER(C.c_uint8), C.c_char_p, C.c_int, C.c_int]),
+    "gd_write_png_rgb": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_uint8)]),
     "gd_default_problem_coefficients": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, _FP * 12]),
     "gd_default_mz_table": (C.c_int, [C.c_size_t, _FP]),
     "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
@@ -213,6 +217,30 @@ def gaussian_pulse_table(nsteps, t0=30.0, tau=10.0, f_cps=0.05):
     out = np.empty(nsteps, np.float32)
     _check_host(lib().gd_gaussian_pulse_table(nsteps, t0, tau, f_cps, _fp(out)))
     return out
+
+
+def field_rgb(u, vmax):
+    """The reference's field colormap: (ny, nx) array -> (ny, nx, 3) uint8."""
+    a = np.ascontiguousarray(u, np.float32)
+    ny, nx = a.shape
+    out = np.empty((ny, nx, 3), np.uint8)
+    _check_host(lib().gd_field_rgb(_fp(a), nx, ny, float(vmax),
+                                   out.ctypes.data_as(C.POINTER(C.c_uint8))), "gd_field_rgb")
+    return out
+
+
+def save_field_png(u, filename, vmax):
+    a = np.ascontiguousarray(u, np.float32)
+    ny, nx = a.shape
+    _check_host(lib().gd_save_field_png(_fp(a), str(filename).encode(), nx, ny, float(vmax)),
+                "gd_save_field_png")
+
+
+def save_mask_png(mask, filename):
+    m = np.ascontiguousarray(mask, np.uint8)
+    ny, nx = m.shape
+    _check_host(lib().gd_save_mask_png(m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                       str(filename).encode(), nx, ny), "gd_save_mask_png")
 
 
 def iso_cb_from_eps(eps_r, dt, dx):

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -171,4 +172,135 @@ extern "C" int gd_gaussian_pulse_table(size_t nsteps, double t0, double tau, dou
     out[n] = (float)(std::exp(-(u * u)) * std::sin(two_pi * f_cps * (double)n));
   }
   return GD_OK;
+}
+
+// ---- figure output ------------------------------------------------------------
+//
+// Pixel rule of the reference's field figures (save_field_png, utils.h:210-269):
+// value = u / vmax in float, clamped to [-1, 1]; value >= 0 interpolates white
+// -> red (r = 255, g = b = (uint8)(255 * (1 - value))), value < 0 white -> blue
+// (b = 255, r = g = (uint8)(255 * (1 + value))); pixel (i, j) is u[i + j*nx],
+// image nx wide and ny high, row j. Mask figures (save_mask_png, utils.h:170-207):
+// black where the mask is set, white elsewhere. The PNG encoder is this
+// library's own: 8-bit RGB, unfiltered rows, zlib stream of stored deflate
+// blocks (valid for every PNG reader; larger files than the reference's
+// compressed ones, identical pixels).
+
+namespace {
+
+uint32_t crc32_update(uint32_t crc, const unsigned char* p, size_t n) {
+  static uint32_t table[256];
+  static bool init = false;
+  if (!init) {
+    for (uint32_t i = 0; i < 256; ++i) {
+      uint32_t c = i;
+      for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+      table[i] = c;
+    }
+    init = true;
+  }
+  for (size_t q = 0; q < n; ++q) crc = table[(crc ^ p[q]) & 0xffu] ^ (crc >> 8);
+  return crc;
+}
+
+void put_be32(std::vector<unsigned char>& v, uint32_t x) {
+  v.push_back((unsigned char)(x >> 24));
+  v.push_back((unsigned char)(x >> 16));
+  v.push_back((unsigned char)(x >> 8));
+  v.push_back((unsigned char)x);
+}
+
+void put_chunk(std::vector<unsigned char>& out, const char type[4],
+               const std::vector<unsigned char>& data) {
+  put_be32(out, (uint32_t)data.size());
+  const size_t start = out.size();
+  out.insert(out.end(), type, type + 4);
+  out.insert(out.end(), data.begin(), data.end());
+  const uint32_t crc =
+      crc32_update(0xffffffffu, out.data() + start, data.size() + 4) ^ 0xffffffffu;
+  put_be32(out, crc);
+}
+
+}  // namespace
+
+extern "C" int gd_field_rgb(const float* u, int nx, int ny, float vmax, uint8_t* rgb) {
+  if (!u || !rgb || nx < 1 || ny < 1) return GD_ERR_ARG;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      const int q = i + j * nx;
+      float value = u[q] / vmax;
+      unsigned char r, g, b;
+      if (value >= 0) {
+        if (value > 1) value = 1;
+        r = 255;
+        g = b = (unsigned char)(255 * (1 - value));
+      } else {
+        if (value < -1) value = -1;
+        b = 255;
+        r = g = (unsigned char)(255 * (1 + value));
+      }
+      rgb[3 * q] = r;
+      rgb[3 * q + 1] = g;
+      rgb[3 * q + 2] = b;
+    }
+  return GD_OK;
+}
+
+extern "C" int gd_write_png_rgb(const char* filename, int w, int h, const uint8_t* rgb) {
+  if (!filename || !rgb || w < 1 || h < 1) return GD_ERR_ARG;
+  std::vector<unsigned char> raw;  // filter byte 0 + row
+  raw.reserve((size_t)h * (3 * (size_t)w + 1));
+  for (int y = 0; y < h; ++y) {
+    raw.push_back(0);
+    raw.insert(raw.end(), rgb + (size_t)y * 3 * w, rgb + (size_t)(y + 1) * 3 * w);
+  }
+  std::vector<unsigned char> z = {0x78, 0x01};
+  size_t pos = 0;
+  do {
+    const size_t n = std::min<size_t>(65535, raw.size() - pos);
+    z.push_back(pos + n == raw.size() ? 1 : 0);  // BFINAL, BTYPE 00 (stored)
+    z.push_back((unsigned char)(n & 0xff));
+    z.push_back((unsigned char)(n >> 8));
+    z.push_back((unsigned char)(~n & 0xff));
+    z.push_back((unsigned char)((~n >> 8) & 0xff));
+    z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + n);
+    pos += n;
+  } while (pos < raw.size());
+  uint32_t a = 1, b = 0;  // Adler-32
+  for (unsigned char c : raw) {
+    a = (a + c) % 65521u;
+    b = (b + a) % 65521u;
+  }
+  put_be32(z, (b << 16) | a);
+  std::vector<unsigned char> png = {0x89, 'P', 'N', 'G', 0x0d, 0x0a, 0x1a, 0x0a};
+  std::vector<unsigned char> ihdr;
+  put_be32(ihdr, (uint32_t)w);
+  put_be32(ihdr, (uint32_t)h);
+  ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit RGB, deflate, no filter, no interlace
+  put_chunk(png, "IHDR", ihdr);
+  put_chunk(png, "IDAT", z);
+  put_chunk(png, "IEND", {});
+  FILE* f = std::fopen(filename, "wb");
+  if (!f) return GD_ERR_ARG;
+  const size_t wrote = std::fwrite(png.data(), 1, png.size(), f);
+  const int closed = std::fclose(f);
+  return wrote == png.size() && closed == 0 ? GD_OK : GD_ERR_ARG;
+}
+
+extern "C" int gd_save_field_png(const float* u, const char* filename, int nx, int ny,
+                                 float vmax) {
+  if (!u || !filename || nx < 1 || ny < 1) return GD_ERR_ARG;
+  std::vector<uint8_t> rgb((size_t)nx * ny * 3);
+  const int rc = gd_field_rgb(u, nx, ny, vmax, rgb.data());
+  return rc != GD_OK ? rc : gd_write_png_rgb(filename, nx, ny, rgb.data());
+}
+
+extern "C" int gd_save_mask_png(const uint8_t* mask, const char* filename, int nx, int ny) {
+  if (!mask || !filename || nx < 1 || ny < 1) return GD_ERR_ARG;
+  std::vector<uint8_t> rgb((size_t)nx * ny * 3);
+  for (size_t q = 0; q < (size_t)nx * ny; ++q) {
+    const uint8_t v = mask[q] ? 0 : 255;
+    rgb[3 * q] = rgb[3 * q + 1] = rgb[3 * q + 2] = v;
+  }
+  return gd_write_png_rgb(filename, nx, ny, rgb.data());
 }

@@ -348,6 +348,20 @@ int gd_default_mz_table(size_t nsteps, float* out);
 int gd_sphere_medium(size_t nx, size_t ny, size_t nz, uint64_t seed, int nspheres,
                      double rmin, double rmax, double emin, double emax, float* eps_out);
 
+/* Figures. gd_field_rgb: the reference's field colormap (save_field_png,
+ * utils.h:210-269): value = u/vmax clamped to [-1,1], white -> red for
+ * positive, white -> blue for negative values, pixel (i,j) = u[i + j*nx];
+ * rgb: 3*nx*ny bytes, row j. gd_save_field_png writes that image as a PNG
+ * (nx wide, ny high); gd_save_mask_png writes black where mask != 0, white
+ * elsewhere (save_mask_png, utils.h:170-207); gd_write_png_rgb writes any
+ * 8-bit RGB image. The PNG encoder is the library's own (stored deflate
+ * blocks): the files differ from the reference's stb output, the decoded
+ * pixels are the same. */
+int gd_field_rgb(const float* u, int nx, int ny, float vmax, uint8_t* rgb);
+int gd_save_field_png(const float* u, const char* filename, int nx, int ny, float vmax);
+int gd_save_mask_png(const uint8_t* mask, const char* filename, int nx, int ny);
+int gd_write_png_rgb(const char* filename, int w, int h, const uint8_t* rgb);
+
 /* Gaussian-modulated sine J(n) = exp(-((n-t0)/tau)^2) * sin(2*pi*f*n),
  * evaluated in double, rounded to float once, n = 0..nsteps-1. */
 int gd_gaussian_pulse_table(size_t nsteps, double t0, double tau, double f_cps, float* out);

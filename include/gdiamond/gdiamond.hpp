@@ -32,7 +32,12 @@
 //     the MI355X tiling is chosen by the library's planner.
 //   * update_FDTD_omp fills sized result vectors (the reference's are never
 //     sized and the call crashes, gdiamond.hpp:37-40).
-//   * Figure output (PNG slices) is not written; the DFT sums are returned by
+//   * The *_figures engines write the reference's PNG files (same names,
+//     cadence, colormap and value scale; gd_save_field_png) but create their
+//     directory when it is missing instead of exiting when it exists
+//     (gdiamond.hpp:1062-1067), and take the Hz slice at i + j*Nx + k*Nx*Ny
+//     where the reference indexes i + j*Nz + k*Nx*Ny (gdiamond.hpp:1139, the
+//     same cell when Nx == Nz). The DFT sums are also returned by
 //     dft_real()/dft_imag().
 #ifndef GDIAMOND_GDIAMOND_HPP
 #define GDIAMOND_GDIAMOND_HPP
@@ -40,6 +45,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <filesystem>
+#include <functional>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -74,6 +81,15 @@ class gDiamond {
   // ---- CPU engines (gdiamond.hpp:1164-1263, gdiamond_omp.hpp:110-172)
   void update_FDTD_seq(size_t num_timesteps) { cpu_engine(num_timesteps, 1, _seq); }
   void update_FDTD_omp(size_t num_timesteps) { cpu_engine(num_timesteps, 0, _omp); }
+  // The same engines writing the Hz mid-plane every T/10 steps
+  // (gdiamond.hpp:1060-1158, gdiamond_omp.hpp:20-110):
+  // seq_figures/Hz_seq_%04ld.png, omp_figures/Hz_omp_%04ld.png.
+  void update_FDTD_seq_figures(size_t num_timesteps) {
+    hz_figures(num_timesteps, 1, _seq, "seq_figures", "Hz_seq");
+  }
+  void update_FDTD_omp_figures(size_t num_timesteps) {
+    hz_figures(num_timesteps, 0, _omp, "omp_figures", "Hz_omp");
+  }
 
   // ---- GPU engines
   void update_FDTD_gpu_2D(size_t num_timesteps) {
@@ -90,7 +106,10 @@ class gDiamond {
   }
   // Gaussian-modulated Jx at the centre and the running DFT of Ex on the
   // k = Nz/2 plane at 26 frequencies 0.90..1.15 f0, scale 50/T
-  // (gdiamond_gpu.cuh:24-281).
+  // (gdiamond_gpu.cuh:24-281). When t % (T/20) == 99 after step t it writes
+  // the Ex mid-plane (value scale 1) and the real and imaginary DFT planes at
+  // frequency 11 (scale 10) as gpu_figures/Ex_naive_gpu_%04ld.png,
+  // fft_Ex_real_%04ld.png, fft_Ex_imag_%04ld.png with t+1 in the name.
   void update_FDTD_gpu_figures(size_t num_timesteps) {
     std::vector<float> jx(num_timesteps);
     for (size_t t = 0; t < num_timesteps; ++t)
@@ -110,9 +129,30 @@ class gDiamond {
     check(gd_dft_config(c, 1u << GD_EX, 2, _Nz / 2, f_cps.data(), f_cps.size(),
                         50.0 / (double)num_timesteps),
           "gd_dft_config", c);
-    run_on_device(num_timesteps, GD_VARIANT_PLAIN, GD_EX, jx, _gpu);
     _dft_re.resize((size_t)nf * _Nx * _Ny);
     _dft_im.resize(_dft_re.size());
+    const size_t every = num_timesteps / 20;
+    std::vector<size_t> stops;  // steps after which figures are written
+    if (every > 99)
+      for (size_t t = 99; t < num_timesteps; t += every) stops.push_back(t + 1);
+    if (!stops.empty()) std::filesystem::create_directories("gpu_figures");
+    auto figures = [&](size_t done) {
+      const size_t P = _Nx * _Ny;
+      std::vector<float> plane(P);
+      check(gd_get_plane_xy(c, GD_EX, _Nz / 2, plane.data()), "gd_get_plane_xy", c);
+      char name[64];
+      std::snprintf(name, sizeof(name), "gpu_figures/Ex_naive_gpu_%04zu.png", done);
+      write_png(plane, name, 1.0f);
+      check(gd_dft_get(c, GD_EX, _dft_re.data(), _dft_im.data()), "gd_dft_get", c);
+      const int fq = 11;
+      for (size_t q = 0; q < P; ++q) plane[q] = (float)_dft_re[(size_t)fq * P + q];
+      std::snprintf(name, sizeof(name), "gpu_figures/fft_Ex_real_%04zu.png", done);
+      write_png(plane, name, 10.0f);
+      for (size_t q = 0; q < P; ++q) plane[q] = (float)_dft_im[(size_t)fq * P + q];
+      std::snprintf(name, sizeof(name), "gpu_figures/fft_Ex_imag_%04zu.png", done);
+      write_png(plane, name, 10.0f);
+    };
+    run_on_device(num_timesteps, GD_VARIANT_PLAIN, GD_EX, jx, _gpu, stops, figures);
     check(gd_dft_get(c, GD_EX, _dft_re.data(), _dft_im.data()), "gd_dft_get", c);
     check(gd_dft_disable(c), "gd_dft_disable", c);
   }
@@ -213,8 +253,10 @@ class gDiamond {
     run_on_device(T, variant, GD_HZ, mz, out);
   }
 
+  // Runs T steps; after each step count in `stops` (increasing) calls at(step).
   void run_on_device(size_t T, int variant, int comp, const std::vector<float>& table,
-                     Set& out) {
+                     Set& out, const std::vector<size_t>& stops = {},
+                     const std::function<void(size_t)>& at = nullptr) {
     gd_ctx* c = ctx();
     check(gd_reset(c), "gd_reset", c);
     check(gd_clear_sources(c), "gd_clear_sources", c);
@@ -223,7 +265,14 @@ class gDiamond {
                    k = _source_idx / (_Nx * _Ny);
       check(gd_add_source(c, comp, i, j, k, table.data(), T), "gd_add_source", c);
     }
-    check(gd_run(c, variant, T, nullptr), "gd_run", c);
+    size_t done = 0;
+    for (size_t s : stops) {
+      if (s > T || s <= done) continue;
+      check(gd_run(c, variant, s - done, nullptr), "gd_run", c);
+      done = s;
+      if (at) at(done);
+    }
+    check(gd_run(c, variant, T - done, nullptr), "gd_run", c);
     float* p[6];
     for (int a = 0; a < 6; ++a) {
       out[a].assign(_N, 0.0f);
@@ -237,7 +286,33 @@ class gDiamond {
   // The default problem's J, Mx and My are zero; the reference subtracts
   // 0*dx for them, which leaves every value unchanged bit for bit, so those
   // terms are omitted.
-  void cpu_engine(size_t T, int threads, Set& out) {
+  static void write_png(const std::vector<float>& plane, const char* name, float vmax,
+                        size_t nx, size_t ny) {
+    if (gd_save_field_png(plane.data(), name, (int)nx, (int)ny, vmax) != GD_OK)
+      std::perror("Failed to write image");
+  }
+  void write_png(const std::vector<float>& plane, const char* name, float vmax) const {
+    write_png(plane, name, vmax, _Nx, _Ny);
+  }
+
+  void hz_figures(size_t T, int threads, Set& out, const char* dir, const char* stem) {
+    std::filesystem::create_directories(dir);
+    const size_t every = T / 10;
+    const float vmax = (float)(1.0 / std::sqrt(1.0f / 1.0f));  // 1/sqrt(mu0/eps0), utils.h:19-20
+    cpu_engine(T, threads, out, [&](size_t t, const float* Hz) {
+      if (every == 0 || t % every != 0) return;
+      std::vector<float> plane(_Nx * _Ny);
+      const size_t k = _Nz / 2;
+      for (size_t j = 0; j < _Ny; ++j)
+        for (size_t i = 0; i < _Nx; ++i) plane[i + j * _Nx] = Hz[i + j * _Nx + k * _Nx * _Ny];
+      char name[64];
+      std::snprintf(name, sizeof(name), "%s/%s_%04zu.png", dir, stem, t);
+      write_png(plane, name, vmax);
+    });
+  }
+
+  void cpu_engine(size_t T, int threads, Set& out,
+                  const std::function<void(size_t, const float*)>& after_step = nullptr) {
     for (auto& v : out) v.assign(_N, 0.0f);
     float *Ex = out[0].data(), *Ey = out[1].data(), *Ez = out[2].data();
     float *Hx = out[3].data(), *Hy = out[4].data(), *Hz = out[5].data();
@@ -267,6 +342,7 @@ class gDiamond {
             Hy[q] = C[8][q] * Hy[q] + C[9][q] * ((Ez[q + 1] - Ez[q]) - (Ex[q + sxy] - Ex[q]));
             Hz[q] = C[10][q] * Hz[q] + C[11][q] * ((Ex[q + Nx] - Ex[q]) - (Ey[q + 1] - Ey[q]) - m * _dx);
           }
+      if (after_step) after_step(t, Hz);
     }
   }
 

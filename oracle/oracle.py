@@ -340,4 +340,7 @@ def ref_lib():
     L.ref_seq.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_float)]
     L.ref_default_mz.argtypes = [C.c_int]
     L.ref_default_mz.restype = C.c_float
+    L.ref_save_field_png.argtypes = [C.POINTER(C.c_float), C.c_char_p, C.c_int, C.c_int,
+                                     C.c_float]
+    L.ref_save_mask_png.argtypes = [C.POINTER(C.c_uint8), C.c_char_p, C.c_int, C.c_int]
     return L

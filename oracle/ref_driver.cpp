@@ -55,4 +55,20 @@ float ref_default_mz(int t) {
   return g.M_source_amp * std::sin(g.SOURCE_OMEGA * (size_t)t * g.dt);
 }
 
+// The reference's figure writers (utils.h:168-269, stb_image_write as the
+// reference vendors it): the PNG a field or a mask becomes.
+int ref_save_field_png(const float* u, const char* filename, int nx, int ny, float vmax) {
+  std::vector<float> v(u, u + (size_t)nx * ny);
+  save_field_png(v.data(), filename, nx, ny, vmax);
+  return 0;
+}
+
+int ref_save_mask_png(const unsigned char* mask, const char* filename, int nx, int ny) {
+  std::vector<char> b((size_t)nx * ny);
+  bool* m = reinterpret_cast<bool*>(b.data());
+  for (size_t q = 0; q < b.size(); ++q) m[q] = mask[q] != 0;
+  save_mask_png(m, filename, nx, ny);
+  return 0;
+}
+
 }  // extern "C"

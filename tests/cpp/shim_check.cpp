@@ -1,6 +1,8 @@
 // Exercises the C++ drop-in include/gdiamond/gdiamond.hpp the way the
 // reference's own test and examples drive gdiamond::gDiamond:
 //   cpu  : seq engine output for the golden-vector comparison, omp == seq
+//   seqfig: update_FDTD_seq_figures in the current directory (figure files)
+//   gpufig: update_FDTD_gpu_figures(2100) in the current directory
 //   gpu  : unittests/checker_gpu.cu:9-22 (seq vs gpu_2D at 100^3 x 100),
 //          examples/gpu_mm_ver2.cu / gpu_mm_ver3.cu (naive vs tiled, constant
 //          source), examples/gpu_figures.cu (DFT monitor)
@@ -36,6 +38,26 @@ int main(int argc, char** argv) {
       }
       std::fclose(f);
       std::printf("cpu ok\n");
+      return 0;
+    }
+    if (mode == "seqfig") {
+      if (argc != 7) return fail("usage: shim_check seqfig NX NY NZ T OUT");
+      const size_t nx = std::atoi(argv[2]), ny = std::atoi(argv[3]), nz = std::atoi(argv[4]);
+      const size_t T = std::atoi(argv[5]);
+      gdiamond::gDiamond exp(nx, ny, nz);
+      exp.update_FDTD_seq_figures(T);
+      FILE* f = std::fopen(argv[6], "wb");
+      if (!f) return fail("cannot open output");
+      const auto& hz = exp.field(gdiamond::gDiamond::SEQ, 5);
+      std::fwrite(hz.data(), sizeof(float), hz.size(), f);
+      std::fclose(f);
+      std::printf("seqfig ok\n");
+      return 0;
+    }
+    if (mode == "gpufig") {
+      gdiamond::gDiamond exp(60, 50, 40);
+      exp.update_FDTD_gpu_figures(2100);
+      std::printf("gpufig ok\n");
       return 0;
     }
     if (mode == "gpu") {

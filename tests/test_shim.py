@@ -44,3 +44,22 @@ def test_shim_gpu_engines(exe):
     assert r.returncode == 0, r.stdout + r.stderr
     for tag in ("checker_gpu ok", "gpu_mm_ver2 ok", "gpu_mm_ver3 ok", "gpu_figures ok"):
         assert tag in r.stdout
+
+
+@pytest.mark.gpu
+def test_shim_gpu_figures_files(exe, tmp_path):
+    """update_FDTD_gpu_figures(2100) writes the reference's figure set: after
+    steps 100, 205, ... (t % (T/20) == 99) the Ex mid-plane and the real and
+    imaginary DFT planes at frequency 11."""
+    from png_read import read_png
+    print("[shim] gpufig", flush=True)
+    r = subprocess.run([exe, "gpufig"], capture_output=True, text=True, timeout=240,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps = [t + 1 for t in range(99, 2100, 105)]
+    names = sorted(os.listdir(tmp_path / "gpu_figures"))
+    want = sorted([f"{p}_{s:04d}.png" for s in steps
+                   for p in ("Ex_naive_gpu", "fft_Ex_real", "fft_Ex_imag")])
+    assert names == want
+    img = read_png(tmp_path / "gpu_figures" / f"fft_Ex_real_{steps[-1]:04d}.png")
+    assert img.shape == (50, 60, 3) and (img != 255).any()
