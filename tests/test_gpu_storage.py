@@ -279,7 +279,7 @@ def test_storage_error_vs_f64(gd):
         err[name] = spectral_error(re, im, ref_re, ref_im)
         say(f"{name}: spectrum error per (component, frequency) {np.array2string(err[name], formatter={'float_kind': lambda x: f'{x:.2e}'})}")
     assert err["f32"].max() < 1e-5
-    # rounding once per launch of 4 steps (TILED) instead of every half step
+    # rounding once per launch of 3 steps (TILED) instead of every half step
     # (PLAIN) lowers the error
     assert err["f16"].max() < 1e-3
     assert err["f16-plain"].max() < 3e-3
