@@ -44,14 +44,16 @@ namespace gd {
 namespace {
 
 // Lane l receives the value of lane l-1 (lane 0 receives 0): wave_shr:1.
+// bound_ctrl writes 0 for the lane without a source, which lets the compiler
+// fold the shift into the subtraction that consumes it (v_sub*_f32_dpp).
 __device__ __forceinline__ float from_lane_below(float v) {
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
 }
 // Lane l receives the value of lane l+1 (lane 63 receives 0): wave_shl:1.
 __device__ __forceinline__ float from_lane_above(float v) {
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs. Give each XCD a
@@ -67,13 +69,19 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 
 // Load / store at a per-plane base (uniform, SGPRs) plus a 32-bit byte offset
 // (per lane), so the hardware's scalar-base + vector-offset addressing is used
-// instead of 64-bit per-lane address arithmetic.
+// instead of 64-bit per-lane address arithmetic. The offsets are computed
+// once before the stream loop; without the empty asm the compiler carries
+// them into the loop as 64-bit values and no longer sees that the high half
+// is zero, so every 4- or 8-byte access in the loop would cost a 64-bit add.
+// (For 2-byte elements the plain form measures faster.)
 template <class S>
 __device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
+  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
   return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
 }
 template <class S>
 __device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
+  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
   *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
 }
 
@@ -336,11 +344,14 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
       // ---- H of step n on plane zH from E(level b) on zH (eq[b]) and zH+1 (en)
       float hn[R][3];
       {
-        float ez_last = 0.0f, ex_last = 0.0f;
-        if (w + 1 < W) {
-          ez_last = lds_e[b - 1][zH & 1][w + 1][0][lane];
-          ex_last = lds_e[b - 1][zH & 1][w + 1][1][lane];
-        }
+        // read unconditionally (the last wave reads its own row) and select,
+        // so that the level stays one basic block and the lane shifts fold
+        // into their subtractions
+        const int wn = w + 1 < W ? w + 1 : w;
+        const float ez_n = lds_e[b - 1][zH & 1][wn][0][lane];
+        const float ex_n = lds_e[b - 1][zH & 1][wn][1][lane];
+        const float ez_last = w + 1 < W ? ez_n : 0.0f;
+        const float ex_last = w + 1 < W ? ex_n : 0.0f;
         const size_t oH = (size_t)clampi(zH, 0, nz - 1) * plane_bytes;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -444,8 +455,11 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
         }
       if (b < B || B == 1) __syncthreads();
       if (b < B) {
-        hmz = w > 0 ? lds_h[b - 1][p & 1][w - 1][0][lane] : 0.0f;
-        hmx = w > 0 ? lds_h[b - 1][p & 1][w - 1][1][lane] : 0.0f;
+        const int wp = w > 0 ? w - 1 : 0;
+        const float hz_p = lds_h[b - 1][p & 1][wp][0][lane];
+        const float hx_p = lds_h[b - 1][p & 1][wp][1][lane];
+        hmz = w > 0 ? hz_p : 0.0f;
+        hmx = w > 0 ? hx_p : 0.0f;
       }
     }
   }
