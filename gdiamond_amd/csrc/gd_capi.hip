@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gdiamond/gd_capi.h"
@@ -118,6 +119,29 @@ int alloc_fields(gd_ctx* c, int copy) {
 
 float scale_of(const gd_ctx* c, int comp) { return std::ldexp(1.0f, c->scale_exp[comp]); }
 
+// Runs f(begin, end) over [0, n) split across up to 16 host threads (one
+// range when n is small). Used for the element-wise format conversions of
+// whole fields, which are independent per element.
+template <class F>
+void parallel_for(size_t n, F&& f) {
+  constexpr size_t kMinChunk = size_t(1) << 20;
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(nt, 16u));
+  nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, n / kMinChunk));
+  if (nt <= 1) {
+    f(size_t(0), n);
+    return;
+  }
+  const size_t chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b < e) th.emplace_back([&f, b, e] { f(b, e); });
+  }
+  f(size_t(0), std::min(n, chunk));
+  for (auto& x : th) x.join();
+}
+
 // Host values -> storage bytes of one component (fp32 host values).
 void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<unsigned char>& out) {
   out.resize(c->N * c->esz);
@@ -125,11 +149,15 @@ void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<un
     std::memcpy(out.data(), in, c->N * sizeof(float));
   } else if (c->storage == GD_STORAGE_F64) {
     double* o = reinterpret_cast<double*>(out.data());
-    for (size_t q = 0; q < c->N; ++q) o[q] = (double)in[q];
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) o[q] = (double)in[q];
+    });
   } else {
     const float inv = std::ldexp(1.0f, -c->scale_exp[comp]);
     uint16_t* o = reinterpret_cast<uint16_t*>(out.data());
-    for (size_t q = 0; q < c->N; ++q) o[q] = gd::f32_to_f16(in[q] * inv);
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) o[q] = gd::f32_to_f16(in[q] * inv);
+    });
   }
 }
 
@@ -137,13 +165,17 @@ void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<un
 void decode_to_f64(const gd_ctx* c, int comp, const unsigned char* in, double* out, size_t n) {
   if (c->storage == GD_STORAGE_F32) {
     const float* v = reinterpret_cast<const float*>(in);
-    for (size_t q = 0; q < n; ++q) out[q] = v[q];
+    parallel_for(n, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) out[q] = v[q];
+    });
   } else if (c->storage == GD_STORAGE_F64) {
     std::memcpy(out, in, n * sizeof(double));
   } else {
     const float s = scale_of(c, comp);
     const uint16_t* v = reinterpret_cast<const uint16_t*>(in);
-    for (size_t q = 0; q < n; ++q) out[q] = (double)(gd::f16_to_f32(v[q]) * s);
+    parallel_for(n, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) out[q] = (double)(gd::f16_to_f32(v[q]) * s);
+    });
   }
 }
 
@@ -466,10 +498,14 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
     std::memcpy(buf.data(), cb, c->N * sizeof(float));
   } else if (c->storage == GD_STORAGE_F64) {
     double* o = reinterpret_cast<double*>(buf.data());
-    for (size_t q = 0; q < c->N; ++q) o[q] = (double)cb[q];
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) o[q] = (double)cb[q];
+    });
   } else {
     uint16_t* o = reinterpret_cast<uint16_t*>(buf.data());
-    for (size_t q = 0; q < c->N; ++q) o[q] = gd::f32_to_f16(cb[q]);
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) o[q] = gd::f32_to_f16(cb[q]);
+    });
   }
   GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
   c->db = db;
@@ -590,10 +626,21 @@ int gd_get_field(gd_ctx* c, int comp, float* out) {
     GD_HIP(c, hipMemcpy(out, c->f[c->cur][comp], c->N * sizeof(float), hipMemcpyDeviceToHost));
     return GD_OK;
   }
-  std::vector<double> v(c->N);
-  int rc = gd_get_field_f64(c, comp, v.data());
-  if (rc != GD_OK) return rc;
-  for (size_t q = 0; q < c->N; ++q) out[q] = (float)v[q];
+  std::vector<unsigned char> buf(c->N * c->esz);
+  GD_HIP(c, hipMemcpy(buf.data(), c->f[c->cur][comp], c->N * c->esz, hipMemcpyDeviceToHost));
+  if (c->storage == GD_STORAGE_F16) {
+    // binary16 times a power of two is exact in float
+    const float sc = scale_of(c, comp);
+    const uint16_t* v = reinterpret_cast<const uint16_t*>(buf.data());
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) out[q] = gd::f16_to_f32(v[q]) * sc;
+    });
+  } else {
+    const double* v = reinterpret_cast<const double*>(buf.data());
+    parallel_for(c->N, [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) out[q] = (float)v[q];
+    });
+  }
   return GD_OK;
 }
 

@@ -220,10 +220,12 @@ float or_f16_to_f32(uint16_t h) {
 }
 
 void or_f32_to_f16_n(const float* in, uint16_t* out, int64_t n) {
+#pragma omp parallel for schedule(static) if (n > (1 << 22))
   for (int64_t q = 0; q < n; ++q) out[q] = or_f32_to_f16(in[q]);
 }
 
 void or_f16_to_f32_n(const uint16_t* in, float* out, int64_t n) {
+#pragma omp parallel for schedule(static) if (n > (1 << 22))
   for (int64_t q = 0; q < n; ++q) out[q] = or_f16_to_f32(in[q]);
 }
 
@@ -262,14 +264,18 @@ static void dft_acc_f16(or_dft* d, int64_t nx, int64_t ny, int64_t nz, uint16_t*
 static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
                           const float* sc, const float* inv, const float* cb, float db,
                           int nsrc, const or_source* src, int64_t n0, int64_t nsteps,
-                          or_dft* dft, int64_t block) {
+                          or_dft* dft, int64_t block, int nth) {
   const int64_t N = nx * ny * nz;
   float* W[6] = {0};
   int rc = 0;
   for (int c = 0; c < 6; ++c) {
     W[c] = (float*)malloc((size_t)N * sizeof(float));
     if (!W[c]) { rc = -2; goto done; }
-    for (int64_t q = 0; q < N; ++q) W[c][q] = dec16(F[c][q], sc[c]);
+    float* w = W[c];
+    const uint16_t* f = F[c];
+    const float s = sc[c];
+#pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
+    for (int64_t q = 0; q < N; ++q) w[q] = dec16(f[q], s);
   }
   or_material m;
   memset(&m, 0, sizeof(m));
@@ -278,13 +284,18 @@ static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t
   m.db = db;
   for (int64_t n = n0; n < n0 + nsteps;) {
     const int64_t nb = n0 + nsteps - n < block ? n0 + nsteps - n : block;
-    rc = or_run_f32(nx, ny, nz, dx, W, &m, nsrc, src, n, nb, dft, 1);
+    rc = or_run_f32(nx, ny, nz, dx, W, &m, nsrc, src, n, nb, dft, nth);
     if (rc) goto done;
-    for (int c = 0; c < 6; ++c)
+    for (int c = 0; c < 6; ++c) {
+      float* w = W[c];
+      uint16_t* f = F[c];
+      const float s = sc[c], iv = inv[c];
+#pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
       for (int64_t q = 0; q < N; ++q) {
-        F[c][q] = enc16(W[c][q], inv[c]);
-        W[c][q] = dec16(F[c][q], sc[c]);
+        f[q] = enc16(w[q], iv);
+        w[q] = dec16(f[q], s);
       }
+    }
     n += nb;
   }
 done:
@@ -295,15 +306,17 @@ done:
 int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
                const int scale_exp[6], const float* cb, float db,
                int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
-               int64_t block) {
+               int64_t block, int threads) {
   if (nx < 3 || ny < 3 || nz < 3 || block < 0) return -1;
+  const int nth = threads > 1 ? threads : 1;
   float sc[6], inv[6];
   for (int c = 0; c < 6; ++c) {
     sc[c] = ldexpf(1.0f, scale_exp ? scale_exp[c] : 0);
     inv[c] = ldexpf(1.0f, scale_exp ? -scale_exp[c] : 0);
   }
   if (block > 0)
-    return run_f16_blocks(nx, ny, nz, dx, F, sc, inv, cb, db, nsrc, src, n0, nsteps, dft, block);
+    return run_f16_blocks(nx, ny, nz, dx, F, sc, inv, cb, db, nsrc, src, n0, nsteps, dft, block,
+                          nth);
   uint16_t *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
   const int64_t sxy = nx * ny;
   for (int64_t n = n0; n < n0 + nsteps; ++n) {

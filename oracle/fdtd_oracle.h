@@ -81,13 +81,16 @@ int or_run_f64(int64_t nx, int64_t ny, int64_t nz, double dx,
  *               shorter), as a temporally blocked update that keeps the
  *               intermediate steps in float registers; the DFT sees the float
  *               value of every step (before the block-end rounding).
- * Fields hold binary16 bit patterns. Isotropic material only. */
+ * Fields hold binary16 bit patterns. Isotropic material only. `threads`
+ * host threads run the block mode (its float steps and the conversions);
+ * every value is computed by the same operations whatever the count. */
 int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const fields[6],
                const int scale_exp[6], const float* cb, float db,
                int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
-               int64_t block);
+               int64_t block, int threads);
 
-/* binary16 conversions used by or_run_f16 (round to nearest even). */
+/* binary16 conversions used by or_run_f16 (round to nearest even); the
+ * array forms split long arrays over the OpenMP threads. */
 uint16_t or_f32_to_f16(float f);
 float or_f16_to_f32(uint16_t h);
 void or_f32_to_f16_n(const float* in, uint16_t* out, int64_t n);

@@ -83,7 +83,7 @@ def lib():
                                                    C.POINTER(C.c_int), C.POINTER(C.c_float),
                                                    C.c_float, C.c_int, C.POINTER(_Source),
                                                    C.c_int64, C.c_int64, C.POINTER(_Dft),
-                                                   C.c_int64]
+                                                   C.c_int64, C.c_int]
         L.or_f32_to_f16.restype = C.c_uint16
         L.or_f32_to_f16.argtypes = [C.c_float]
         L.or_f16_to_f32.restype = C.c_float
@@ -222,12 +222,13 @@ def _dft_struct(dft, shape, keep):
 
 
 def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps=1,
-            dft=None, block=0):
+            dft=None, block=0, threads=1):
     """FP16-storage twin: fields16 are 6 uint16 (binary16 bit) arrays shaped
     (nz, ny, nx), advanced in place; cb the float32 Cb array (exact values, or
     the decoded binary16 coefficient of the dense 16-bit form). ``block`` 0
     rounds after every half step (untiled update), block B after every B steps
-    (temporally blocked update with B levels per launch)."""
+    (temporally blocked update with B levels per launch); ``threads`` host
+    threads run the block mode, with results independent of the count."""
     nx, ny, nz = shape
     sh = (nz, ny, nx)
     for f in fields16:
@@ -249,7 +250,7 @@ def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps
     P16 = C.POINTER(C.c_uint16)
     fa = (P16 * 6)(*[f.ctypes.data_as(P16) for f in fields16])
     rc = lib().or_run_f16(nx, ny, nz, float(dx), fa, se, _fp(cb), float(db),
-                          nsrc, src_arr, n0, nsteps, dptr, int(block))
+                          nsrc, src_arr, n0, nsteps, dptr, int(block), int(threads))
     if rc != 0:
         raise RuntimeError("oracle run failed")
     return out

@@ -171,3 +171,25 @@ def test_run_f16_block_differs_from_half_step(oracle):
     oracle.run_f16(shape, a, se, cb, 0.5, sources=src, nsteps=T, block=0)
     oracle.run_f16(shape, b, se, cb, 0.5, sources=src, nsteps=T, block=1)
     assert any(not np.array_equal(a[c], b[c]) for c in range(6))
+
+
+def test_run_f16_block_threads_do_not_change_results(oracle):
+    """The threaded block mode (used for the large FP16 checks) == one thread,
+    fields and DFT bit for bit; the threaded conversions of long arrays ==
+    numpy's float16 casts."""
+    se = [-3, -2, -2, 1, 0, 2]
+    T = 11
+    shape, cb, src, a = _small_problem(oracle, T, se)
+    b = [x.copy() for x in a]
+    dft = dict(comps=[0, 5], axis=2, index=4, f_cps=np.array([0.05]), scale=1.0)
+    r1 = oracle.run_f16(shape, a, se, cb, 0.5, sources=src, nsteps=T, dft=dft, block=3)
+    r4 = oracle.run_f16(shape, b, se, cb, 0.5, sources=src, nsteps=T, dft=dft, block=3,
+                        threads=4)
+    for c in range(6):
+        assert np.array_equal(a[c], b[c]), c
+    assert bits_equal(r1[0], r4[0]) and bits_equal(r1[1], r4[1])
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(5_000_000) * 100).astype(np.float32)  # above the parallel cutoff
+    bits = oracle.f32_to_f16_bits(x)
+    assert np.array_equal(bits, x.astype(np.float16).view(np.uint16))
+    assert np.array_equal(oracle.decode_f16(bits), bits.view(np.float16).astype(np.float32))
