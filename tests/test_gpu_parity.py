@@ -195,7 +195,7 @@ def test_dft_monitor_fp64(gpu, oracle, axis, index):
             assert bits_equal(im, wim[s]), f"variant {variant} comp {c} im: {describe_diff(im, wim[s])}"
 
 
-# ------------------------------------------------------------------ BASELINE configs
+# ------------------------------------------------------- benchmark workloads, full size
 
 
 def test_config_vacuum_128_200_steps(gpu, oracle):
