@@ -245,7 +245,7 @@ def run_learned(args, gd):
     48^3, 8 snapshots each) train the corrector; a blend factor alpha for the
     applied correction is chosen on a held-out problem (fine 128^3 x 400
     steps); then the fine 512^3 x 2000-step run (eps 1-6, seed 6000) is the
-    judge of the coarse 256^3 x 1000-step run with and without the corrector
+    reference for the coarse 256^3 x 1000-step run with and without the corrector
     (every 4 coarse steps): relative DFT-spectrum error of Ex, Ey on the mid
     plane and run time."""
     from gdiamond_amd import learned
@@ -261,9 +261,10 @@ def run_learned(args, gd):
                                  log=lambda s: print("# " + s, file=sys.stderr, flush=True))
     train_s = time.perf_counter() - t0
     del raw
-    out_dir = os.path.join(ROOT, "gpurun_out")
-    os.makedirs(out_dir, exist_ok=True)
-    np.savez(os.path.join(out_dir, "corrector_params.npz"), params=params, cfg=np.asarray(cfg))
+    if args.learned_params_out:
+        d = os.path.dirname(os.path.abspath(args.learned_params_out))
+        os.makedirs(d, exist_ok=True)
+        np.savez(args.learned_params_out, params=params, cfg=np.asarray(cfg))
 
     def with_alpha(a):
         c = list(cfg)
@@ -272,7 +273,7 @@ def run_learned(args, gd):
         return tuple(c)
 
     # blend factor on a held-out problem with the full rollout length (the
-    # same number of corrections as the main run); alpha 0 = corrector off
+    # same number of corrections as the measured run); alpha 0 = corrector off
     val = {}
     nval = 128
     for a in (1.0 / 32, 1.0 / 16, 1.0 / 8, 1.0 / 4, 1.0 / 2, 1.0):
@@ -426,13 +427,15 @@ def main():
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--learned-pairs", type=int, default=64)
     ap.add_argument("--learned-epochs", type=int, default=6)
+    ap.add_argument("--learned-params-out", default="",
+                    help="write the trained corrector parameters to this .npz file")
     ap.add_argument("--batch-timesteps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to calibrate HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
 

@@ -4,7 +4,7 @@
 // kernels its training needs (backward data / weight passes, loss, Adam).
 //
 // No counterpart in the reference (it has no learned component); the fine
-// grid solver stays the judge of its output.
+// grid solver stays the reference its output is measured against.
 //
 // Tensors are channels-last with 16 channels per cell, cell = x + nx*(y + ny*z)
 // (the field layout): one 8-byte load gives a lane the 4 consecutive channels

@@ -235,7 +235,7 @@ def restrict_spectrum(re, im, axis_comp):
 
 def compare(n_fine, nsteps_fine, params, cfg, seed=6000, nspheres=200, rmin=4.0, rmax=20.0,
             f_cps=(0.03, 0.05, 0.07), interval=4, log=None):
-    """Fine run (the judge) against the coarse run without and with the
+    """Fine run (the reference) against the coarse run without and with the
     corrector: relative DFT-spectrum error of Ex and Ey on the mid plane
     (max over components and frequencies) and wall time of each run."""
     prob = Problem(n_fine, seed, nspheres, rmin, rmax, 0.05, nsteps_fine)

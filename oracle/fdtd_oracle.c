@@ -2,7 +2,8 @@
  * CPU oracle for the 3D Yee FDTD time-stepping path -- TEST INFRASTRUCTURE.
  *
  * Only the test-suite, the package smoke check and the benchmark's CPU-baseline
- * leg load this library, always as the checker. The product never links it.
+ * leg load this library, always as the reference results are compared
+ * against. The product never links it.
  *
  * Every function restates the reference algorithm it names; pinning of each
  * function (golden vectors from the reference build, closed forms, invariants)

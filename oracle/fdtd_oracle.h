@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the 3D Yee FDTD time-stepping path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by the
+ * TEST INFRASTRUCTURE ONLY. GPU results are compared against this library: it is loaded by the
  * test-suite, by the package smoke check and by the CPU-baseline leg of the
  * benchmark, never by the product library or the C++ host shim.
  *

@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE.
 
 Imported only by tests/, by ``__graft_entry__.smoke()`` and by the CPU-baseline
-leg of ``bench.py``, always as the checker. The product package never imports
+leg of ``bench.py``, always as the reference the GPU results are compared
+against. The product package never imports
 this module.
 
 Functions mirror the reference CPU path:

@@ -126,7 +126,7 @@ def test_host_builders_reject_bad_args(gd):
 
 
 def test_library_has_no_oracle_dependency(gd):
-    """The product library never links the checker."""
+    """The product library never links the oracle."""
     import subprocess
     out = subprocess.run(["ldd", gd.LIB_PATH], capture_output=True, text=True).stdout
     assert "liboracle" not in out and "gdref" not in out
