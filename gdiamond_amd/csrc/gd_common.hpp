@@ -84,15 +84,23 @@ struct Codec<double> {
 template <>
 struct Codec<_Float16> {
   using T = float;
+  // Both directions are one mixed-precision FMA with a -0 addend: the binary16
+  // operand converts to float exactly, the product by a power of two is exact
+  // for the scales the library accepts (|exponent| <= 100 keeps it normal and
+  // finite), and x + (-0) == x bit for bit, signed zeros included. So dec is
+  // (float)v * s and enc is (_Float16)(v * inv) rounded once to nearest even,
+  // in one instruction each instead of a convert and a multiply. (A +0
+  // addend, which the compiler picks when it fuses the pair by itself, would
+  // turn -0 into +0.)
   __device__ __forceinline__ static T dec(_Float16 v, float s) {
-    float f = (float)v;
-    asm("" : "+v"(f));
-    return f * s;
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(v), "s"(s), "v"(-0.0f));
+    return r;
   }
   __device__ __forceinline__ static _Float16 enc(T v, float inv) {
-    float t = v * inv;
-    asm("" : "+v"(t));
-    return (_Float16)t;
+    _Float16 r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, %3" : "=v"(r) : "v"(v), "s"(inv), "v"(-0.0f));
+    return r;
   }
   __device__ __forceinline__ static T round(T v, float s, float inv) {
     return dec(enc(v, inv), s);
