@@ -403,7 +403,7 @@ def run_batch(args, gd):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed runs of the whole workload")
+    ap.add_argument("--steps", type=int, default=5, help="timed runs of the whole workload")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--timesteps", type=int, default=1000)
@@ -416,7 +416,8 @@ def main():
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--zchunk", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=120,
+                    help="time steps of the CPU baseline sample (about 6-10 s on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-leg", action="store_true",
                     help="skip the fixed-error comparison (FP64 reference, FP32 PLAIN, FP16)")
@@ -561,8 +562,9 @@ def main():
         cpu = cpu_baseline(gd, n, eps, dt, src, args.cpu_steps, threads)
         out["cpu_baseline"] = {"value": round(cpu, 4), "unit": "Gcell-updates/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"same {n}^3 workload, {args.cpu_steps} of {T} steps, "
-                                         f"OpenMP collapse(2) port of update_FDTD_omp"}
+                               "sample": f"same {n}^3 workload, steps 1..{args.cpu_steps} of {T} "
+                                         f"(after one untimed first-touch step), OpenMP "
+                                         f"collapse(2) port of update_FDTD_omp"}
     print(json.dumps(out), flush=True)
 
 
