@@ -153,8 +153,8 @@ def run_precision(args, gd):
     """Precision form: FP16 field storage (scale 2^F16_SCALE_EXP, exact FP32 Cb
     through the indexed material), FP32 arithmetic, FP64 DFT, TILED kernel.
     Throughput on the 1024^3 sphere medium (seed 43) for 2000 steps; error of
-    the FP16 and FP32 forms against an FP64 run of the identical problem at
-    256^3 (same generator and seed, 2000 steps)."""
+    the FP16, BF16 and FP32 forms against an FP64 run of the identical problem
+    at 256^3 (same generator and seed, 2000 steps)."""
     T = 2000
     dt = np.float32(0.99 / math.sqrt(3.0))
     src = gd.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
@@ -164,6 +164,8 @@ def run_precision(args, gd):
         ctx = gd.Context(n, n, n, dx=1.0)
         if storage == gd.F16:
             ctx.set_storage(gd.F16, [F16_SCALE_EXP] * 6)
+        elif storage == gd.BF16:
+            ctx.set_storage(gd.BF16)  # float's exponent range: no scale needed
         elif storage == gd.F64:
             ctx.set_storage(gd.F64)
         ctx.set_isotropic_eps(eps, dt)
@@ -178,7 +180,8 @@ def run_precision(args, gd):
     res = {}
     ref = None
     for name, storage, variant in (("f64-plain", gd.F64, gd.PLAIN), ("f32-tiled", gd.F32, gd.TILED),
-                                   ("f16-tiled", gd.F16, gd.TILED), ("f16-plain", gd.F16, gd.PLAIN)):
+                                   ("f16-tiled", gd.F16, gd.TILED), ("f16-plain", gd.F16, gd.PLAIN),
+                                   ("bf16-tiled", gd.BF16, gd.TILED)):
         ctx = make(ne, storage, eps_s)
         ctx.run(T, variant)
         spec = spectra(ctx, gd)

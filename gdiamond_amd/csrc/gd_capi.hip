@@ -119,6 +119,15 @@ int alloc_fields(gd_ctx* c, int copy) {
 
 float scale_of(const gd_ctx* c, int comp) { return std::ldexp(1.0f, c->scale_exp[comp]); }
 
+// 16-bit storage formats (binary16, bfloat16) and their element conversions.
+bool is16(int storage) { return storage == GD_STORAGE_F16 || storage == GD_STORAGE_BF16; }
+uint16_t enc16(int storage, float v) {
+  return storage == GD_STORAGE_BF16 ? gd::f32_to_bf16(v) : gd::f32_to_f16(v);
+}
+float dec16(int storage, uint16_t h) {
+  return storage == GD_STORAGE_BF16 ? gd::bf16_to_f32(h) : gd::f16_to_f32(h);
+}
+
 // Runs f(begin, end) over [0, n) split across up to 16 host threads (one
 // range when n is small). Used for the element-wise format conversions of
 // whole fields, which are independent per element.
@@ -154,9 +163,10 @@ void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<un
     });
   } else {
     const float inv = std::ldexp(1.0f, -c->scale_exp[comp]);
+    const int st = c->storage;
     uint16_t* o = reinterpret_cast<uint16_t*>(out.data());
     parallel_for(c->N, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) o[q] = gd::f32_to_f16(in[q] * inv);
+      for (size_t q = b; q < e; ++q) o[q] = enc16(st, in[q] * inv);
     });
   }
 }
@@ -172,9 +182,10 @@ void decode_to_f64(const gd_ctx* c, int comp, const unsigned char* in, double* o
     std::memcpy(out, in, n * sizeof(double));
   } else {
     const float s = scale_of(c, comp);
+    const int st = c->storage;
     const uint16_t* v = reinterpret_cast<const uint16_t*>(in);
     parallel_for(n, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) out[q] = (double)(gd::f16_to_f32(v[q]) * s);
+      for (size_t q = b; q < e; ++q) out[q] = (double)(dec16(st, v[q]) * s);
     });
   }
 }
@@ -279,7 +290,7 @@ gd_tiling plan(const gd_ctx* c) {
   // Four levels per launch for 4- and 8-byte fields. With binary16 fields the
   // HBM share of a launch is half as large and three levels run faster
   // (227 vs 185 Gcell/s at 512^3: less overlap, no register spills).
-  t.blt = c->storage == GD_STORAGE_F16 ? 3 : 4;
+  t.blt = is16(c->storage) ? 3 : 4;
   t.rows = 4;
   t.waves = 8;
   const size_t ox = 64 - 2 * t.blt, oy = (size_t)(t.rows * t.waves - 2 * t.blt);
@@ -291,7 +302,7 @@ gd_tiling plan(const gd_ctx* c) {
   // chunk halves the recomputed z halo, worth it while the grid still gives
   // at least 16 workgroups per CU (1024^3: 258 vs 251 Gcell/s; the FP32 form
   // measured slower with longer chunks)
-  if (c->storage == GD_STORAGE_F16 && txy * ((c->nz + 127) / 128) >= 4096) zc = 128;
+  if (is16(c->storage) && txy * ((c->nz + 127) / 128) >= 4096) zc = 128;
   if (zc > c->nz) zc = c->nz;
   t.zchunk = (int)zc;
   return t;
@@ -385,7 +396,7 @@ const char* gd_last_error(const gd_ctx* c) {
 
 int gd_set_storage(gd_ctx* c, int storage, const int* scale_exp) {
   if (!c) return GD_ERR_ARG;
-  if (storage != GD_STORAGE_F32 && storage != GD_STORAGE_F64 && storage != GD_STORAGE_F16)
+  if (storage != GD_STORAGE_F32 && storage != GD_STORAGE_F64 && !is16(storage))
     return c->fail(GD_ERR_ARG, "unknown storage format");
   if (c->have_mat) return c->fail(GD_ERR_STATE, "set the storage format before the material");
   int se[6] = {0, 0, 0, 0, 0, 0};
@@ -404,7 +415,7 @@ int gd_set_storage(gd_ctx* c, int storage, const int* scale_exp) {
     }
   c->storage = storage;
   std::memcpy(c->scale_exp, se, sizeof(se));
-  c->esz = storage == GD_STORAGE_F64 ? 8 : (storage == GD_STORAGE_F16 ? 2 : 4);
+  c->esz = storage == GD_STORAGE_F64 ? 8 : (is16(storage) ? 2 : 4);
   c->cur = 0;
   c->step = 0;
   int rc = alloc_fields(c, 0);
@@ -427,7 +438,7 @@ int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
   if (!coeff) return c->fail(GD_ERR_ARG, "null coefficient table");
   for (int a = 0; a < 12; ++a)
     if (!coeff[a]) return c->fail(GD_ERR_ARG, "null coefficient array");
-  if (c->storage == GD_STORAGE_F16)
+  if (is16(c->storage))
     return c->fail(GD_ERR_UNSUPPORTED, "16-bit storage supports the isotropic material only");
   free_material(c);
   const size_t es = c->storage == GD_STORAGE_F64 ? sizeof(double) : sizeof(float);
@@ -478,7 +489,7 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   // dense FP32 Cb otherwise (the same results; the dense tiled kernel is the
   // faster one for FP32 fields)
   const bool want_index = c->mat_req == GD_MATERIAL_INDEXED ||
-                          (c->mat_req == GD_MATERIAL_AUTO && c->storage == GD_STORAGE_F16);
+                          (c->mat_req == GD_MATERIAL_AUTO && is16(c->storage));
   if (want_index) {
     if (c->storage == GD_STORAGE_F64) {
       if (c->mat_req == GD_MATERIAL_INDEXED)
@@ -507,9 +518,10 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
       for (size_t q = b; q < e; ++q) o[q] = (double)cb[q];
     });
   } else {
+    const int st = c->storage;
     uint16_t* o = reinterpret_cast<uint16_t*>(buf.data());
     parallel_for(c->N, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) o[q] = gd::f32_to_f16(cb[q]);
+      for (size_t q = b; q < e; ++q) o[q] = enc16(st, cb[q]);
     });
   }
   GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
@@ -633,12 +645,13 @@ int gd_get_field(gd_ctx* c, int comp, float* out) {
   }
   std::vector<unsigned char> buf(c->N * c->esz);
   GD_HIP(c, hipMemcpy(buf.data(), c->f[c->cur][comp], c->N * c->esz, hipMemcpyDeviceToHost));
-  if (c->storage == GD_STORAGE_F16) {
-    // binary16 times a power of two is exact in float
+  if (is16(c->storage)) {
+    // a 16-bit value times a power of two is exact in float
     const float sc = scale_of(c, comp);
+    const int st = c->storage;
     const uint16_t* v = reinterpret_cast<const uint16_t*>(buf.data());
     parallel_for(c->N, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) out[q] = gd::f16_to_f32(v[q]) * sc;
+      for (size_t q = b; q < e; ++q) out[q] = dec16(st, v[q]) * sc;
     });
   } else {
     const double* v = reinterpret_cast<const double*>(buf.data());

@@ -5,8 +5,8 @@
 // update (gdiamond.hpp:1193-1221):
 //   E = Ca*E + Cb*(((Hp - Hq) - (Hr - Hs)) - J*dx)
 //   H = Da*H + Db*(((Ep - Eq) - (Er - Es)) - M*dx)
-// computed in the storage format's compute type (FP32 for FP32 and FP16
-// storage, FP64 for FP64 storage), so results are bit-identical to the
+// computed in the storage format's compute type (FP32 for FP32, FP16 and
+// BF16 storage, FP64 for FP64 storage), so results are bit-identical to the
 // sequential CPU update of the same storage format.
 #pragma once
 
@@ -25,7 +25,9 @@ constexpr int kMaxBlt = 8;
 //         decoded exactly to float, computed in float and rounded to binary16
 //         (round to nearest even) when stored: after every half step in the
 //         PLAIN kernels, once per launch of B steps in the TILED kernel.
-enum Storage { kF32 = 0, kF64 = 1, kF16 = 2 };
+//   kBF16: bfloat16 (the upper half of a float) of value * 2^-k_c, same
+//         decode / compute / round-when-stored rules as kF16.
+enum Storage { kF32 = 0, kF64 = 1, kF16 = 2, kBF16 = 3 };
 
 // Field arrays of one field copy (SoA, idx = i + j*nx + k*nx*ny); the element
 // type follows the storage format.
@@ -76,11 +78,7 @@ struct Codec<double> {
   __device__ __forceinline__ static double enc(T v, float) { return v; }
   __device__ __forceinline__ static T round(T v, float, float) { return v; }
 };
-// binary16 storage of value * 2^-k. The empty asm statements keep the scaling
-// multiply and the conversion apart: left alone, the compiler fuses them into
-// v_fma_mix* with a +0 addend, which turns a product that rounds to -0 into +0
-// (and a decoded -0 into +0), so results would differ from IEEE
-// multiply-then-convert in the sign of zero.
+// binary16 storage of value * 2^-k.
 template <>
 struct Codec<_Float16> {
   using T = float;
@@ -101,6 +99,25 @@ struct Codec<_Float16> {
     _Float16 r;
     asm("v_fma_mixlo_f16 %0, %1, %2, %3" : "=v"(r) : "v"(v), "s"(inv), "v"(-0.0f));
     return r;
+  }
+  __device__ __forceinline__ static T round(T v, float s, float inv) {
+    return dec(enc(v, inv), s);
+  }
+};
+// bfloat16 storage of value * 2^-k: decode widens the 16 stored bits (the
+// upper half of a float) exactly and multiplies by 2^k in float; encode
+// multiplies by 2^-k in float and rounds to the nearest bfloat16, ties to even
+// (v_cvt_pk_bf16_f32). bfloat16 has float's exponent range, so the scale only
+// matters for values near the ends of that range.
+template <>
+struct Codec<__bf16> {
+  using T = float;
+  __device__ __forceinline__ static T dec(__bf16 v, float s) {
+    const uint32_t u = (uint32_t)__builtin_bit_cast(uint16_t, v) << 16;
+    return __builtin_bit_cast(float, u) * s;
+  }
+  __device__ __forceinline__ static __bf16 enc(T v, float inv) {
+    return (__bf16)(v * inv);
   }
   __device__ __forceinline__ static T round(T v, float s, float inv) {
     return dec(enc(v, inv), s);

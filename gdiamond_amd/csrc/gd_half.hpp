@@ -1,6 +1,6 @@
-// Host-side IEEE binary16 conversions (round to nearest, ties to even; with
-// subnormals, overflow to infinity, NaN kept quiet), matching the device's
-// v_cvt_f16_f32 / v_cvt_f32_f16 in their default modes.
+// Host-side IEEE binary16 and bfloat16 conversions (round to nearest, ties to
+// even; with subnormals, overflow to infinity, NaN kept quiet), matching the
+// device's conversions in their default modes.
 #pragma once
 
 #include <cstdint>
@@ -55,6 +55,24 @@ inline float f16_to_f32(uint16_t h) {
     }
     x = sign | ((uint32_t)(113 - s) << 23) | ((m & 0x3ffu) << 13);
   }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+// bfloat16 (the upper 16 bits of a float): round to nearest, ties to even, on
+// the bit pattern (subnormals and overflow to infinity follow from the carry;
+// NaN kept quiet), as the device's v_cvt_pk_bf16_f32.
+inline uint16_t f32_to_bf16(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+
+inline float bf16_to_f32(uint16_t h) {
+  const uint32_t x = (uint32_t)h << 16;
   float f;
   std::memcpy(&f, &x, 4);
   return f;

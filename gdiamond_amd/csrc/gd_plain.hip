@@ -11,7 +11,7 @@
 // from registers. The DFT monitor of E components is fused into the E kernel
 // (the value after step n is final there); H components into the H kernel.
 //
-// Templated on the storage element S (float, double or _Float16 with a
+// Templated on the storage element S (float, double, or _Float16 / __bf16 with a
 // per-component power-of-two scale; arithmetic runs in Codec<S>::T) and on the
 // material kind (dense isotropic Cb, the reference's 12 arrays, or a 1-byte
 // material id into a small FP32 Cb table that stays in cache).
@@ -234,6 +234,10 @@ hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, cons
       if (mk == kMatFull) return hipErrorInvalidValue;
       if (mk == kMatIdx) return launch_pair<_Float16, kMatIdx>(f, sc, m, g, s, d, n, run_step, stream);
       return launch_pair<_Float16, kMatIso>(f, sc, m, g, s, d, n, run_step, stream);
+    case kBF16:
+      if (mk == kMatFull) return hipErrorInvalidValue;
+      if (mk == kMatIdx) return launch_pair<__bf16, kMatIdx>(f, sc, m, g, s, d, n, run_step, stream);
+      return launch_pair<__bf16, kMatIso>(f, sc, m, g, s, d, n, run_step, stream);
   }
   return hipErrorInvalidValue;
 }

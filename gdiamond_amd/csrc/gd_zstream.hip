@@ -581,6 +581,11 @@ hipError_t launch_one(const ZArgs& a, Storage st, MatKind mk, hipStream_t stream
       launch_k<B, R, W, kMatIdx, _Float16>(a, stream);
     else
       launch_k<B, R, W, kMatIso, _Float16>(a, stream);
+  } else if (st == kBF16) {
+    if (mk == kMatIdx)
+      launch_k<B, R, W, kMatIdx, __bf16>(a, stream);
+    else
+      launch_k<B, R, W, kMatIso, __bf16>(a, stream);
   } else if (mk == kMatFull) {
     launch_k<B, R, W, kMatFull, float>(a, stream);
   } else if (mk == kMatIdx) {
@@ -631,7 +636,7 @@ hipError_t launch_dft_snapshot(const Dft& d, int nlev, int run_step0, hipStream_
 }
 
 bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk) {
-  if (st == kF64 || (st == kF16 && mk == kMatFull)) return false;
+  if (st == kF64 || ((st == kF16 || st == kBF16) && mk == kMatFull)) return false;
   if (t.blt < 1 || t.blt > 4 || t.zchunk < 1) return false;
   const bool rw = (t.rows == 2 && t.waves == 4) || (t.rows == 2 && t.waves == 8) ||
                   (t.rows == 4 && t.waves == 4) || (t.rows == 4 && t.waves == 8);

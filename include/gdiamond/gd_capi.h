@@ -81,7 +81,8 @@ typedef struct gd_tiling {
 enum {
   GD_STORAGE_F32 = 0, /* float fields, float arithmetic: the reference's format */
   GD_STORAGE_F64 = 1, /* double fields and arithmetic (PLAIN only): reference runs */
-  GD_STORAGE_F16 = 2  /* binary16 of value*2^-k per component; float arithmetic */
+  GD_STORAGE_F16 = 2, /* binary16 of value*2^-k per component; float arithmetic */
+  GD_STORAGE_BF16 = 3 /* bfloat16 of value*2^-k per component; float arithmetic */
 };
 
 /* ---- lifecycle --------------------------------------------------------- */
@@ -109,7 +110,10 @@ const char* gd_version(void);
  * nsteps % blt steps). The DFT accumulates the stored values (PLAIN) or the
  * FP32 value of every step (TILED). It halves the field bytes per cell-update
  * (24 B instead of 48 B; the material adds 1 B indexed, 2 B dense). No counterpart in the
- * reference, which stores FP32 only (gdiamond.hpp:799-891). */
+ * reference, which stores FP32 only (gdiamond.hpp:799-891).
+ * GD_STORAGE_BF16 follows the same rules with bfloat16 (the upper half of a
+ * float: 8 significand bits against binary16's 11, float's exponent range)
+ * rounded to nearest even. */
 int gd_set_storage(gd_ctx* ctx, int storage, const int* scale_exp);
 int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
 
@@ -118,7 +122,7 @@ int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
 /* Device formats of the isotropic material (gd_set_material_format), and what
  * gd_get_material_format reports. */
 enum {
-  GD_MATERIAL_AUTO = 0,    /* GD_STORAGE_F16: INDEXED when Cb has at most GD_MAX_MATERIALS
+  GD_MATERIAL_AUTO = 0,    /* 16-bit storage: INDEXED when Cb has at most GD_MAX_MATERIALS
                               distinct values, else DENSE; other storage formats: DENSE */
   GD_MATERIAL_DENSE = 1,   /* one Cb per cell in the field storage format */
   GD_MATERIAL_INDEXED = 2, /* one 1-byte material id per cell + an FP32 table of Cb values */
@@ -129,8 +133,9 @@ enum {
 /* Choose how the isotropic Cb is held on the device; must precede the
  * material (default GD_MATERIAL_AUTO). INDEXED keeps Cb exact in FP32 for
  * every field storage format and costs 1 B per cell-update instead of 4 (FP32)
- * or 2 (FP16); with GD_STORAGE_F16 the DENSE form stores Cb as binary16, whose
- * rounding changes the medium (relative Cb error up to 2^-11). INDEXED is not
+ * or 2 (16-bit); with 16-bit storage the DENSE form stores Cb as binary16 or
+ * bfloat16, whose rounding changes the medium (relative Cb error up to 2^-11,
+ * 2^-8). INDEXED is not
  * available with GD_STORAGE_F64. No counterpart in the reference, which
  * streams 12 FP32 coefficient arrays (gdiamond_gpu.cuh:871-887). */
 int gd_set_material_format(gd_ctx* ctx, int format);

@@ -220,6 +220,33 @@ float or_f16_to_f32(uint16_t h) {
   return (float)((h & 0x8000u) ? -v : v);
 }
 
+/* bfloat16 = the upper 16 bits of a float, rounded to nearest even on the bit
+ * pattern (exact integer arithmetic; NaN stays a quiet NaN). */
+uint16_t or_f32_to_bf16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  x += 0x7fffu + ((x >> 16) & 1u);
+  return (uint16_t)(x >> 16);
+}
+
+float or_bf16_to_f32(uint16_t h) {
+  const uint32_t x = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+void or_f32_to_bf16_n(const float* in, uint16_t* out, int64_t n) {
+#pragma omp parallel for schedule(static) if (n > (1 << 22))
+  for (int64_t q = 0; q < n; ++q) out[q] = or_f32_to_bf16(in[q]);
+}
+
+void or_bf16_to_f32_n(const uint16_t* in, float* out, int64_t n) {
+#pragma omp parallel for schedule(static) if (n > (1 << 22))
+  for (int64_t q = 0; q < n; ++q) out[q] = or_bf16_to_f32(in[q]);
+}
+
 void or_f32_to_f16_n(const float* in, uint16_t* out, int64_t n) {
 #pragma omp parallel for schedule(static) if (n > (1 << 22))
   for (int64_t q = 0; q < n; ++q) out[q] = or_f32_to_f16(in[q]);
@@ -230,11 +257,16 @@ void or_f16_to_f32_n(const uint16_t* in, float* out, int64_t n) {
   for (int64_t q = 0; q < n; ++q) out[q] = or_f16_to_f32(in[q]);
 }
 
-static inline float dec16(uint16_t h, float s) { return or_f16_to_f32(h) * s; }
-static inline uint16_t enc16(float v, float inv) { return or_f32_to_f16(v * inv); }
+/* 16-bit storage element <-> float: fmt 0 = binary16, 1 = bfloat16. */
+static inline float dec16(int fmt, uint16_t h, float s) {
+  return (fmt ? or_bf16_to_f32(h) : or_f16_to_f32(h)) * s;
+}
+static inline uint16_t enc16(int fmt, float v, float inv) {
+  return fmt ? or_f32_to_bf16(v * inv) : or_f32_to_f16(v * inv);
+}
 
-static void dft_acc_f16(or_dft* d, int64_t nx, int64_t ny, int64_t nz, uint16_t* const f[6],
-                        const float* sc, int64_t n) {
+static void dft_acc_f16(int fmt, or_dft* d, int64_t nx, int64_t ny, int64_t nz,
+                        uint16_t* const f[6], const float* sc, int64_t n) {
   if (!d || d->ncomp <= 0) return;
   const int64_t P = plane_size(d, nx, ny, nz), sxy = nx * ny;
   for (int c = 0; c < d->ncomp; ++c) {
@@ -252,7 +284,7 @@ static void dft_acc_f16(or_dft* d, int64_t nx, int64_t ny, int64_t nz, uint16_t*
             if (d->axis == 2) { if (k != d->index) continue; p = i + j * nx; }
             else if (d->axis == 1) { if (j != d->index) continue; p = i + k * nx; }
             else { if (i != d->index) continue; p = j + k * ny; }
-            const double e = (double)dec16(a[i + j * nx + k * sxy], sc[comp]);
+            const double e = (double)dec16(fmt, a[i + j * nx + k * sxy], sc[comp]);
             re[p] += e * wc;
             im[p] -= e * ws;
           }
@@ -262,7 +294,8 @@ static void dft_acc_f16(or_dft* d, int64_t nx, int64_t ny, int64_t nz, uint16_t*
 
 /* Block mode of or_run_f16: blocks of `block` steps (the last one shorter)
  * run in float from the decoded state, then the state is rounded to binary16. */
-static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
+static int run_f16_blocks(int fmt, int64_t nx, int64_t ny, int64_t nz, float dx,
+                          uint16_t* const F[6],
                           const float* sc, const float* inv, const float* cb, float db,
                           int nsrc, const or_source* src, int64_t n0, int64_t nsteps,
                           or_dft* dft, int64_t block, int nth) {
@@ -276,7 +309,7 @@ static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t
     const uint16_t* f = F[c];
     const float s = sc[c];
 #pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
-    for (int64_t q = 0; q < N; ++q) w[q] = dec16(f[q], s);
+    for (int64_t q = 0; q < N; ++q) w[q] = dec16(fmt, f[q], s);
   }
   or_material m;
   memset(&m, 0, sizeof(m));
@@ -293,8 +326,8 @@ static int run_f16_blocks(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t
       const float s = sc[c], iv = inv[c];
 #pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
       for (int64_t q = 0; q < N; ++q) {
-        f[q] = enc16(w[q], iv);
-        w[q] = dec16(f[q], s);
+        f[q] = enc16(fmt, w[q], iv);
+        w[q] = dec16(fmt, f[q], s);
       }
     }
     n += nb;
@@ -304,11 +337,11 @@ done:
   return rc;
 }
 
-int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
+int or_run_h16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
                const int scale_exp[6], const float* cb, float db,
                int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
-               int64_t block, int threads) {
-  if (nx < 3 || ny < 3 || nz < 3 || block < 0) return -1;
+               int64_t block, int threads, int fmt) {
+  if (nx < 3 || ny < 3 || nz < 3 || block < 0 || fmt < 0 || fmt > 1) return -1;
   const int nth = threads > 1 ? threads : 1;
   float sc[6], inv[6];
   for (int c = 0; c < 6; ++c) {
@@ -316,8 +349,8 @@ int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6
     inv[c] = ldexpf(1.0f, scale_exp ? -scale_exp[c] : 0);
   }
   if (block > 0)
-    return run_f16_blocks(nx, ny, nz, dx, F, sc, inv, cb, db, nsrc, src, n0, nsteps, dft, block,
-                          nth);
+    return run_f16_blocks(fmt, nx, ny, nz, dx, F, sc, inv, cb, db, nsrc, src, n0, nsteps, dft,
+                          block, nth);
   uint16_t *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
   const int64_t sxy = nx * ny;
   for (int64_t n = n0; n < n0 + nsteps; ++n) {
@@ -325,38 +358,46 @@ int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6
       for (int64_t j = 1; j < ny - 1; j++)
         for (int64_t i = 1; i < nx - 1; i++) {
           const int64_t idx = i + j * nx + k * sxy;
-          const float hz = dec16(Hz[idx], sc[5]), hy = dec16(Hy[idx], sc[4]),
-                      hx = dec16(Hx[idx], sc[3]);
-          const float cx = ((hz - dec16(Hz[idx - nx], sc[5])) - (hy - dec16(Hy[idx - sxy], sc[4]))) -
+          const float hz = dec16(fmt, Hz[idx], sc[5]), hy = dec16(fmt, Hy[idx], sc[4]),
+                      hx = dec16(fmt, Hx[idx], sc[3]);
+          const float cx = ((hz - dec16(fmt, Hz[idx - nx], sc[5])) - (hy - dec16(fmt, Hy[idx - sxy], sc[4]))) -
                            src_val_f(nsrc, src, 0, i, j, k, n) * dx;
-          const float cy = ((hx - dec16(Hx[idx - sxy], sc[3])) - (hz - dec16(Hz[idx - 1], sc[5]))) -
+          const float cy = ((hx - dec16(fmt, Hx[idx - sxy], sc[3])) - (hz - dec16(fmt, Hz[idx - 1], sc[5]))) -
                            src_val_f(nsrc, src, 1, i, j, k, n) * dx;
-          const float cz = ((hy - dec16(Hy[idx - 1], sc[4])) - (hx - dec16(Hx[idx - nx], sc[3]))) -
+          const float cz = ((hy - dec16(fmt, Hy[idx - 1], sc[4])) - (hx - dec16(fmt, Hx[idx - nx], sc[3]))) -
                            src_val_f(nsrc, src, 2, i, j, k, n) * dx;
           const float b = cb[idx];
-          Ex[idx] = enc16(dec16(Ex[idx], sc[0]) + b * cx, inv[0]);
-          Ey[idx] = enc16(dec16(Ey[idx], sc[1]) + b * cy, inv[1]);
-          Ez[idx] = enc16(dec16(Ez[idx], sc[2]) + b * cz, inv[2]);
+          Ex[idx] = enc16(fmt, dec16(fmt, Ex[idx], sc[0]) + b * cx, inv[0]);
+          Ey[idx] = enc16(fmt, dec16(fmt, Ey[idx], sc[1]) + b * cy, inv[1]);
+          Ez[idx] = enc16(fmt, dec16(fmt, Ez[idx], sc[2]) + b * cz, inv[2]);
         }
     for (int64_t k = 1; k < nz - 1; k++)
       for (int64_t j = 1; j < ny - 1; j++)
         for (int64_t i = 1; i < nx - 1; i++) {
           const int64_t idx = i + j * nx + k * sxy;
-          const float ex = dec16(Ex[idx], sc[0]), ey = dec16(Ey[idx], sc[1]),
-                      ez = dec16(Ez[idx], sc[2]);
-          const float cx = ((dec16(Ey[idx + sxy], sc[1]) - ey) - (dec16(Ez[idx + nx], sc[2]) - ez)) -
+          const float ex = dec16(fmt, Ex[idx], sc[0]), ey = dec16(fmt, Ey[idx], sc[1]),
+                      ez = dec16(fmt, Ez[idx], sc[2]);
+          const float cx = ((dec16(fmt, Ey[idx + sxy], sc[1]) - ey) - (dec16(fmt, Ez[idx + nx], sc[2]) - ez)) -
                            src_val_f(nsrc, src, 3, i, j, k, n) * dx;
-          const float cy = ((dec16(Ez[idx + 1], sc[2]) - ez) - (dec16(Ex[idx + sxy], sc[0]) - ex)) -
+          const float cy = ((dec16(fmt, Ez[idx + 1], sc[2]) - ez) - (dec16(fmt, Ex[idx + sxy], sc[0]) - ex)) -
                            src_val_f(nsrc, src, 4, i, j, k, n) * dx;
-          const float cz = ((dec16(Ex[idx + nx], sc[0]) - ex) - (dec16(Ey[idx + 1], sc[1]) - ey)) -
+          const float cz = ((dec16(fmt, Ex[idx + nx], sc[0]) - ex) - (dec16(fmt, Ey[idx + 1], sc[1]) - ey)) -
                            src_val_f(nsrc, src, 5, i, j, k, n) * dx;
-          Hx[idx] = enc16(dec16(Hx[idx], sc[3]) + db * cx, inv[3]);
-          Hy[idx] = enc16(dec16(Hy[idx], sc[4]) + db * cy, inv[4]);
-          Hz[idx] = enc16(dec16(Hz[idx], sc[5]) + db * cz, inv[5]);
+          Hx[idx] = enc16(fmt, dec16(fmt, Hx[idx], sc[3]) + db * cx, inv[3]);
+          Hy[idx] = enc16(fmt, dec16(fmt, Hy[idx], sc[4]) + db * cy, inv[4]);
+          Hz[idx] = enc16(fmt, dec16(fmt, Hz[idx], sc[5]) + db * cz, inv[5]);
         }
-    dft_acc_f16(dft, nx, ny, nz, F, sc, n);
+    dft_acc_f16(fmt, dft, nx, ny, nz, F, sc, n);
   }
   return 0;
+}
+
+int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const F[6],
+               const int scale_exp[6], const float* cb, float db,
+               int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
+               int64_t block, int threads) {
+  return or_run_h16(nx, ny, nz, dx, F, scale_exp, cb, db, nsrc, src, n0, nsteps, dft, block,
+                    threads, 0);
 }
 
 /* Literal reference loop with J/M arrays (gdiamond.hpp:1164-1221): the arrays

@@ -89,6 +89,20 @@ int or_run_f16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const fie
                int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
                int64_t block, int threads);
 
+/* The same for a 16-bit format fmt: 0 = binary16 (== or_run_f16), 1 =
+ * bfloat16 (the upper half of a float, rounded to nearest even). */
+int or_run_h16(int64_t nx, int64_t ny, int64_t nz, float dx, uint16_t* const fields[6],
+               const int scale_exp[6], const float* cb, float db,
+               int nsrc, const or_source* src, int64_t n0, int64_t nsteps, or_dft* dft,
+               int64_t block, int threads, int fmt);
+
+/* bfloat16 conversions used by or_run_h16 (round to nearest even on the bit
+ * pattern, NaN kept quiet). */
+uint16_t or_f32_to_bf16(float f);
+float or_bf16_to_f32(uint16_t h);
+void or_f32_to_bf16_n(const float* in, uint16_t* out, int64_t n);
+void or_bf16_to_f32_n(const uint16_t* in, float* out, int64_t n);
+
 /* binary16 conversions used by or_run_f16 (round to nearest even); the
  * array forms split long arrays over the OpenMP threads. */
 uint16_t or_f32_to_f16(float f);

@@ -85,6 +85,16 @@ def lib():
                                                    C.c_float, C.c_int, C.POINTER(_Source),
                                                    C.c_int64, C.c_int64, C.POINTER(_Dft),
                                                    C.c_int64, C.c_int]
+        L.or_run_h16.restype = C.c_int
+        L.or_run_h16.argtypes = [C.c_int64] * 3 + [C.c_float, C.POINTER(C.c_uint16) * 6,
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                                   C.c_float, C.c_int, C.POINTER(_Source),
+                                                   C.c_int64, C.c_int64, C.POINTER(_Dft),
+                                                   C.c_int64, C.c_int, C.c_int]
+        L.or_f32_to_bf16_n.restype = None
+        L.or_f32_to_bf16_n.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_uint16), C.c_int64]
+        L.or_bf16_to_f32_n.restype = None
+        L.or_bf16_to_f32_n.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_int64]
         L.or_f32_to_f16.restype = C.c_uint16
         L.or_f32_to_f16.argtypes = [C.c_float]
         L.or_f16_to_f32.restype = C.c_float
@@ -223,13 +233,14 @@ def _dft_struct(dft, shape, keep):
 
 
 def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps=1,
-            dft=None, block=0, threads=1):
+            dft=None, block=0, threads=1, fmt="f16"):
     """FP16-storage twin: fields16 are 6 uint16 (binary16 bit) arrays shaped
     (nz, ny, nx), advanced in place; cb the float32 Cb array (exact values, or
     the decoded binary16 coefficient of the dense 16-bit form). ``block`` 0
     rounds after every half step (untiled update), block B after every B steps
     (temporally blocked update with B levels per launch); ``threads`` host
-    threads run the block mode, with results independent of the count."""
+    threads run the block mode, with results independent of the count.
+    ``fmt`` "bf16": the fields hold bfloat16 bit patterns instead (same rules)."""
     nx, ny, nz = shape
     sh = (nz, ny, nx)
     for f in fields16:
@@ -250,8 +261,11 @@ def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps
         dptr, out = _dft_struct(dft, shape, keep)
     P16 = C.POINTER(C.c_uint16)
     fa = (P16 * 6)(*[f.ctypes.data_as(P16) for f in fields16])
-    rc = lib().or_run_f16(nx, ny, nz, float(dx), fa, se, _fp(cb), float(db),
-                          nsrc, src_arr, n0, nsteps, dptr, int(block), int(threads))
+    if fmt not in ("f16", "bf16"):
+        raise ValueError("fmt must be 'f16' or 'bf16'")
+    rc = lib().or_run_h16(nx, ny, nz, float(dx), fa, se, _fp(cb), float(db),
+                          nsrc, src_arr, n0, nsteps, dptr, int(block), int(threads),
+                          1 if fmt == "bf16" else 0)
     if rc != 0:
         raise RuntimeError("oracle run failed")
     return out
@@ -275,6 +289,27 @@ def decode_f16(bits, scale_exp=0):
     flat = np.ascontiguousarray(b.ravel())
     out = np.empty(flat.size, np.float32)
     lib().or_f16_to_f32_n(flat.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(out), flat.size)
+    return (out * np.float32(2.0 ** scale_exp)).reshape(b.shape)
+
+
+def f32_to_bf16_bits(values):
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    out = np.empty(v.size, np.uint16)
+    lib().or_f32_to_bf16_n(_fp(v), out.ctypes.data_as(C.POINTER(C.c_uint16)), v.size)
+    return out
+
+
+def encode_bf16(values, scale_exp=0):
+    """Storage encoding of float values: bfloat16 of value * 2^-scale_exp."""
+    v = np.ascontiguousarray(values, np.float32) * np.float32(2.0 ** -scale_exp)
+    return f32_to_bf16_bits(v).reshape(np.shape(values))
+
+
+def decode_bf16(bits, scale_exp=0):
+    b = np.ascontiguousarray(bits, np.uint16)
+    flat = np.ascontiguousarray(b.ravel())
+    out = np.empty(flat.size, np.float32)
+    lib().or_bf16_to_f32_n(flat.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(out), flat.size)
     return (out * np.float32(2.0 ** scale_exp)).reshape(b.shape)
 
 

@@ -2,13 +2,13 @@
 
 Each case draws, from its own seed: a grid (3..130 x 3..90 x 3..50 cells), a
 tiling (levels per launch, rows x waves, z chunk; or the planner's choice),
-field storage (FP32, or binary16 with per-component scales) and material
+field storage (FP32, or binary16 / bfloat16 with per-component scales) and material
 (dense or indexed isotropic Cb, or the reference's 12 coefficient arrays for
 FP32), up to three point sources, zero or random initial fields, a DFT plane
 (any axis, a subset of components, 1-3 frequencies) and a run split into up to
 three calls. The fields and the FP64 DFT sums must equal the oracle's:
-`oracle.run` for FP32, the blocked binary16 twin `oracle.run_f16` (rounding
-once per launch of `blt` steps) for FP16.
+`oracle.run` for FP32, the blocked 16-bit twin `oracle.run_f16` (rounding
+once per launch of `blt` steps) for the 16-bit formats.
 """
 import numpy as np
 import pytest
@@ -26,7 +26,9 @@ def _draw(gd, seed):
     rng = np.random.default_rng(5000 + seed)
     nx, ny, nz = int(rng.integers(3, 131)), int(rng.integers(3, 91)), int(rng.integers(3, 51))
     shape = (nx, ny, nz)
-    f16 = rng.random() < 0.3
+    r = rng.random()
+    fmt = "bf16" if r < 0.12 else ("f16" if r < 0.3 else None)  # 16-bit storage
+    f16 = fmt is not None
     mat = rng.choice(["dense", "indexed"] if f16 else ["dense", "indexed", "full"])
     if rng.random() < 0.2:
         tiling = None
@@ -70,7 +72,7 @@ def _draw(gd, seed):
     cuts = sorted(set(int(c) for c in rng.integers(1, total + 1, size=int(rng.integers(0, 3)))))
     chunks = [b - a for a, b in zip([0] + cuts, cuts + [total]) if b > a]
     scale = [int(v) for v in rng.integers(-6, 4, size=6)] if f16 else None
-    return dict(shape=shape, f16=f16, mat=str(mat), tiling=tiling, cb=cb, db=db, coeffs=coeffs,
+    return dict(shape=shape, f16=f16, fmt=fmt, mat=str(mat), tiling=tiling, cb=cb, db=db, coeffs=coeffs,
                 sources=sources, init=init, dft=dft, chunks=chunks, scale=scale)
 
 
@@ -78,7 +80,7 @@ def _gpu_run(gd, case):
     ctx = gd.Context(*case["shape"])
     try:
         if case["f16"]:
-            ctx.set_storage(gd.F16, case["scale"])
+            ctx.set_storage(gd.BF16 if case["fmt"] == "bf16" else gd.F16, case["scale"])
         if case["coeffs"] is not None:
             ctx.set_coefficients(case["coeffs"])
         else:
@@ -114,24 +116,26 @@ def _oracle_run(oracle, case, blt):
                                           dft=d, threads=8, **kw)
             n0 += n
         return f, (d["re"], d["im"])
-    se = case["scale"]
+    se, fmt = case["scale"], case["fmt"]
+    enc, dec = ((oracle.encode_bf16, oracle.decode_bf16) if fmt == "bf16" else
+                (oracle.encode_f16, oracle.decode_f16))
     init = case["init"] or [np.zeros((nz, ny, nx), np.float32)] * 6
-    bits = [oracle.encode_f16(init[c], se[c]) for c in range(6)]
+    bits = [enc(init[c], se[c]) for c in range(6)]
     cb = case["cb"]
-    if case["mat"] == "dense":  # binary16 coefficient of the dense 16-bit form
-        cb = oracle.decode_f16(oracle.encode_f16(cb, 0))
+    if case["mat"] == "dense":  # 16-bit coefficient of the dense 16-bit form
+        cb = dec(enc(cb, 0))
     n0 = 0
     for n in case["chunks"]:
         d["re"], d["im"] = oracle.run_f16(shape, bits, se, cb, case["db"], sources=case["sources"],
-                                          n0=n0, nsteps=n, dft=d, block=blt)
+                                          n0=n0, nsteps=n, dft=d, block=blt, fmt=fmt)
         n0 += n
-    return [oracle.decode_f16(bits[c], se[c]) for c in range(6)], (d["re"], d["im"])
+    return [dec(bits[c], se[c]) for c in range(6)], (d["re"], d["im"])
 
 
 @pytest.mark.parametrize("seed", range(NCASES))
 def test_random_case_matches_oracle(gd, oracle, seed):
     case = _draw(gd, seed)
-    say(f"[random {seed}] shape {case['shape']} f16 {case['f16']} mat {case['mat']} "
+    say(f"[random {seed}] shape {case['shape']} 16-bit {case['fmt']} mat {case['mat']} "
         f"tiling {case['tiling']} sources {len(case['sources'])} chunks {case['chunks']} "
         f"dft axis {case['dft']['axis']} index {case['dft']['index']}")
     fields, spec, blt = _gpu_run(gd, case)

@@ -11,7 +11,8 @@ fixed-error comparison), and the dense / indexed isotropic material.
   default) and the indexed material == the oracle on every tiling.
 * FP64 PLAIN == oracle.run(dtype=float64) bit for bit (fields and DFT).
 * Material-format selection and errors.
-* Error of FP32 and FP16 storage against FP64 on the same problem.
+* bfloat16 PLAIN and TILED == oracle.run_f16(fmt="bf16") bit for bit.
+* Error of FP32, FP16 and BF16 storage against FP64 on the same problem.
 """
 import numpy as np
 import pytest
@@ -34,26 +35,34 @@ def _f16_problem(gd, shape, T, seed=11):
     return cb, db, src, dft
 
 
-def _cb_effective(oracle, cb, mat):
-    """Cb the kernels compute with: exact for the indexed material, binary16
-    for the dense material of 16-bit storage."""
+def _codec(oracle, fmt):
+    if fmt == "bf16":
+        return oracle.encode_bf16, oracle.decode_bf16
+    return oracle.encode_f16, oracle.decode_f16
+
+
+def _cb_effective(oracle, cb, mat, fmt="f16"):
+    """Cb the kernels compute with: exact for the indexed material, the 16-bit
+    value for the dense material of 16-bit storage."""
+    enc, dec = _codec(oracle, fmt)
     if mat == "dense":
-        return oracle.decode_f16(oracle.encode_f16(cb, 0))
+        return dec(enc(cb, 0))
     return cb
 
 
 def _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, chunks=None, mat="indexed",
-                block=0):
-    f16 = [oracle.encode_f16(init[c], scale_exp[c]) for c in range(6)]
-    cb16 = _cb_effective(oracle, cb, mat)
+                block=0, fmt="f16"):
+    enc, dec = _codec(oracle, fmt)
+    f16 = [enc(init[c], scale_exp[c]) for c in range(6)]
+    cb16 = _cb_effective(oracle, cb, mat, fmt)
     dft = dict(dft)
     n0 = 0
     for n in (chunks or [T]):
         dft["re"], dft["im"] = oracle.run_f16(shape, f16, scale_exp, cb16, db, sources=src,
-                                              n0=n0, nsteps=n, dft=dft, block=block)
+                                              n0=n0, nsteps=n, dft=dft, block=block, fmt=fmt)
         n0 += n
     re, im = dft["re"], dft["im"]
-    vals = [oracle.decode_f16(f16[c], scale_exp[c]) for c in range(6)]
+    vals = [dec(f16[c], scale_exp[c]) for c in range(6)]
     return vals, (re, im)
 
 
@@ -214,6 +223,29 @@ def test_f16_runs_continue(gd, oracle):
         _assert_same(got, want, f"f16 continue {variant}")
 
 
+@pytest.mark.parametrize("mat", ["dense", "indexed"])
+@pytest.mark.parametrize("tiling", [None, TILINGS[2], TILINGS[5], TILINGS[4]],
+                         ids=lambda t: "plain" if t is None else
+                         "b{blt}r{rows}w{waves}z{zchunk}".format(**t))
+def test_bf16_matches_oracle(gd, oracle, tiling, mat):
+    """bfloat16 storage, PLAIN (rounding every half step) and TILED (once per
+    launch) == the oracle's bfloat16 twin, fields and DFT, over a run split in
+    two calls."""
+    shape = (57, 38, 27)
+    T = 21
+    scale_exp = SCALES[1]
+    cb, db, src, dft = _f16_problem(gd, shape, T, seed=13)
+    init = random_fields(shape, 4, scale=0.05, boundary_zero=True)
+    chunks = [9, 12]
+    want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, chunks=chunks,
+                       mat=mat, block=0 if tiling is None else tiling["blt"], fmt="bf16")
+    say("bf16", shape, tiling, mat)
+    got = _gpu(gd, shape, gd.BF16, scale_exp, cb, db, src, T, dft, init=init,
+               variant=gd.PLAIN if tiling is None else gd.TILED, tiling=tiling, mat=mat,
+               chunks=chunks)
+    _assert_same(got, want, f"bf16 {tiling} {mat}")
+
+
 def test_f64_plain_matches_oracle(gd, oracle):
     shape = (39, 31, 26)
     T = 35
@@ -274,7 +306,8 @@ def test_storage_error_vs_f64(gd):
     for name, st, se, mat, var in (("f32", gd.F32, None, "indexed", gd.TILED),
                                    ("f16", gd.F16, [-2] * 6, "indexed", gd.TILED),
                                    ("f16-plain", gd.F16, [-2] * 6, "indexed", gd.PLAIN),
-                                   ("f16-dense-cb", gd.F16, [-2] * 6, "dense", gd.TILED)):
+                                   ("f16-dense-cb", gd.F16, [-2] * 6, "dense", gd.TILED),
+                                   ("bf16", gd.BF16, [0] * 6, "indexed", gd.TILED)):
         _, (re, im) = _gpu(gd, shape, st, se, cb, db, src, T, dft, variant=var, mat=mat)
         err[name] = spectral_error(re, im, ref_re, ref_im)
         say(f"{name}: spectrum error per (component, frequency) {np.array2string(err[name], formatter={'float_kind': lambda x: f'{x:.2e}'})}")
@@ -285,3 +318,6 @@ def test_storage_error_vs_f64(gd):
     assert err["f16-plain"].max() < 3e-3
     assert err["f16-dense-cb"].max() < 3e-2
     assert err["f16-dense-cb"].max() > err["f16-plain"].max() > err["f16"].max() > err["f32"].max()
+    # bfloat16 keeps 8 significand bits against binary16's 11: about 2^3 times
+    # the per-rounding error, with the same rounding schedule
+    assert 2 * err["f16"].max() < err["bf16"].max() < 16 * err["f16"].max()

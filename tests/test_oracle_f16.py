@@ -5,6 +5,8 @@
 * or_run_f16 == an independent numpy restatement of the storage semantics
   (decode to float32, float32 arithmetic in the reference operand order,
   astype(float16) once per step), bit for bit.
+* the bfloat16 variant (fmt="bf16") == the same restatement with torch's
+  bfloat16 rounding, and its conversions == torch's casts.
 """
 import numpy as np
 import pytest
@@ -40,9 +42,25 @@ def test_f16_to_f32_matches_numpy(oracle):
     assert bits_equal(got, want)
 
 
-def _numpy_f16_step(f, cb, db, sc, src, n, dx):
-    """One step of the binary16-storage semantics in numpy (independent of the C)."""
-    dec = [f[c].view(np.float16).astype(np.float32) * np.float32(sc[c]) for c in range(6)]
+# 16-bit codecs of the restatements: binary16 through numpy's float16 casts,
+# bfloat16 through torch's (both round to nearest even, independently of the C)
+F16_NP = (lambda v: v.astype(np.float16).view(np.uint16),
+          lambda b: b.view(np.float16).astype(np.float32))
+
+
+def _bf16_torch():
+    torch = pytest.importorskip("torch")
+    to_bits = lambda v: torch.from_numpy(np.ascontiguousarray(v, np.float32)).to(
+        torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    from_bits = lambda b: torch.from_numpy(np.ascontiguousarray(b).view(np.int16)).view(
+        torch.bfloat16).to(torch.float32).numpy()
+    return to_bits, from_bits
+
+
+def _numpy_f16_step(f, cb, db, sc, src, n, dx, codec=F16_NP):
+    """One step of the 16-bit-storage semantics in numpy (independent of the C)."""
+    to_bits, from_bits = codec
+    dec = [from_bits(f[c]) * np.float32(sc[c]) for c in range(6)]
     Ex, Ey, Ez, Hx, Hy, Hz = dec
     b = np.asarray(cb, np.float32)
     I = np.s_[1:-1, 1:-1, 1:-1]
@@ -56,8 +74,8 @@ def _numpy_f16_step(f, cb, db, sc, src, n, dx):
     cz = ((Hy[I] - Hy[im]) - (Hx[I] - Hx[jm])) - J[2] * dxf
     for c, cur in ((0, cx), (1, cy), (2, cz)):
         v = (dec[c][I] + b[I] * cur) * np.float32(1.0 / sc[c])
-        f[c][I] = v.astype(np.float16).view(np.uint16)
-    Ex, Ey, Ez = [f[c].view(np.float16).astype(np.float32) * np.float32(sc[c]) for c in range(3)]
+        f[c][I] = to_bits(v)
+    Ex, Ey, Ez = [from_bits(f[c]) * np.float32(sc[c]) for c in range(3)]
     kp, jp, ip = np.s_[2:, 1:-1, 1:-1], np.s_[1:-1, 2:, 1:-1], np.s_[1:-1, 1:-1, 2:]
     dbf = np.float32(db)
     cx = ((Ey[kp] - Ey[I]) - (Ez[jp] - Ez[I])) - J[3] * dxf
@@ -65,7 +83,7 @@ def _numpy_f16_step(f, cb, db, sc, src, n, dx):
     cz = ((Ex[jp] - Ex[I]) - (Ey[ip] - Ey[I])) - J[5] * dxf
     for c, cur in ((3, cx), (4, cy), (5, cz)):
         v = (dec[c][I] + dbf * cur) * np.float32(1.0 / sc[c])
-        f[c][I] = v.astype(np.float16).view(np.uint16)
+        f[c][I] = to_bits(v)
 
 
 @pytest.mark.parametrize("cb_f16", [True, False], ids=["cb-binary16", "cb-exact"])
@@ -93,10 +111,11 @@ def test_run_f16_matches_numpy_restatement(oracle, scale_exp, cb_f16):
     assert not np.array_equal(a[2], oracle.encode_f16(init[2], scale_exp[2]))
 
 
-def _numpy_block_run(f16, cb, db, sc, src, n0, nsteps, block, dx=1.0):
+def _numpy_block_run(f16, cb, db, sc, src, n0, nsteps, block, dx=1.0, codec=F16_NP):
     """Block-rounding semantics in numpy: float steps from the decoded state,
-    binary16 rounding of the whole state after every `block` steps."""
-    v = [f16[c].view(np.float16).astype(np.float32) * np.float32(sc[c]) for c in range(6)]
+    16-bit rounding of the whole state after every `block` steps."""
+    to_bits, from_bits = codec
+    v = [from_bits(f16[c]) * np.float32(sc[c]) for c in range(6)]
     I = np.s_[1:-1, 1:-1, 1:-1]
     km, jm, im = np.s_[:-2, 1:-1, 1:-1], np.s_[1:-1, :-2, 1:-1], np.s_[1:-1, 1:-1, :-2]
     kp, jp, ip = np.s_[2:, 1:-1, 1:-1], np.s_[1:-1, 2:, 1:-1], np.s_[1:-1, 1:-1, 2:]
@@ -119,8 +138,8 @@ def _numpy_block_run(f16, cb, db, sc, src, n0, nsteps, block, dx=1.0):
         done += 1
         if done % block == 0 or n == n0 + nsteps - 1:
             for c in range(6):
-                f16[c][...] = (v[c] * np.float32(1.0 / sc[c])).astype(np.float16).view(np.uint16)
-                v[c] = f16[c].view(np.float16).astype(np.float32) * np.float32(sc[c])
+                f16[c][...] = to_bits(v[c] * np.float32(1.0 / sc[c]))
+                v[c] = from_bits(f16[c]) * np.float32(sc[c])
 
 
 def _small_problem(oracle, T, scale_exp):
@@ -193,3 +212,61 @@ def test_run_f16_block_threads_do_not_change_results(oracle):
     bits = oracle.f32_to_f16_bits(x)
     assert np.array_equal(bits, x.astype(np.float16).view(np.uint16))
     assert np.array_equal(oracle.decode_f16(bits), bits.view(np.float16).astype(np.float32))
+
+
+def test_f32_to_bf16_matches_torch(oracle):
+    """bfloat16 encoding == torch's float -> bfloat16 cast (round to nearest even)
+    over normal, subnormal, halfway, overflowing and infinite values; decoding
+    is the exact widening."""
+    to_bits, from_bits = _bf16_torch()
+    rng = np.random.default_rng(1)
+    parts = [
+        np.array([0.0, -0.0, 1.0, -1.0, 3.3895314e38, 3.4e38, -3.4028235e38, np.inf, -np.inf,
+                  1e-45, -1e-45, 1.1754942e-38, 1.1754944e-38, 9.2e-41], np.float32),
+        (rng.standard_normal(50000) * np.exp(rng.uniform(-87, 87, 50000))).astype(np.float32),
+        rng.integers(0, 2**32, 50000, dtype=np.uint64).astype(np.uint32).view(np.float32),
+        # exact halfway points between bfloat16 neighbours, both parities
+        ((np.arange(0, 4000, dtype=np.uint32) << 16) | 0x8000).view(np.float32),
+    ]
+    x = np.concatenate(parts)
+    x = x[~np.isnan(x)]
+    got = oracle.f32_to_bf16_bits(x)
+    assert np.array_equal(got, to_bits(x))
+    bits = np.arange(0, 65536, 3, dtype=np.uint16)
+    bits = bits[(bits & 0x7f80) != 0x7f80]  # finite only
+    assert bits_equal(oracle.decode_bf16(bits), from_bits(bits))
+    assert np.all(oracle.f32_to_bf16_bits(np.array([np.nan], np.float32)) & 0x7fc0 == 0x7fc0)
+
+
+@pytest.mark.parametrize("block", [0, 1, 3])
+def test_run_bf16_matches_restatement(oracle, block):
+    """bfloat16 storage, per half step (block 0) and per block of steps ==
+    the numpy restatement with torch's bfloat16 rounding."""
+    codec = _bf16_torch()
+    se = [-3, 0, -2, 1, 0, 2]
+    T = 13
+    shape = (13, 11, 10)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(19)
+    cb = (0.5 / rng.uniform(1.0, 8.0, (nz, ny, nx))).astype(np.float32)
+    src = [(2, *centre(shape), (np.sin(np.arange(T) * 0.4) * 3).astype(np.float32)),
+           (3, 3, 4, 5, (np.cos(np.arange(T) * 0.3)).astype(np.float32))]
+    init = [(rng.standard_normal((nz, ny, nx)) * 0.3).astype(np.float32) for _ in range(6)]
+    a = [oracle.encode_bf16(init[c], se[c]) for c in range(6)]
+    for c in range(6):
+        assert np.array_equal(a[c], codec[0](init[c] * np.float32(2.0 ** -se[c])))
+    b = [x.copy() for x in a]
+    oracle.run_f16(shape, a, se, cb, 0.5, sources=src, nsteps=T, block=block, fmt="bf16")
+    sc = [2.0 ** e for e in se]
+    if block == 0:
+        for n in range(T):
+            _numpy_f16_step(b, cb, 0.5, sc, src, n, 1.0, codec=codec)
+    else:
+        _numpy_block_run(b, cb, 0.5, sc, src, 0, T, block, codec=codec)
+    for c in range(6):
+        assert np.array_equal(a[c], b[c]), c
+    # bfloat16 and binary16 storage give different fields
+    h = [oracle.encode_f16(init[c], se[c]) for c in range(6)]
+    oracle.run_f16(shape, h, se, cb, 0.5, sources=src, nsteps=T, block=block)
+    assert not np.allclose(oracle.decode_bf16(a[2], se[2]), oracle.decode_f16(h[2], se[2]),
+                           rtol=0, atol=0)
