@@ -401,8 +401,8 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
   for (int i = 0; i < ninst; ++i) {
     const size_t nx = dims[3 * i], ny = dims[3 * i + 1], nz = dims[3 * i + 2];
     if (nx < 3 || ny < 3 || nz < 3 || nx > (1u << 20) || ny > (1u << 20) || nz > (1u << 20) ||
-        nx * ny * 4 >= (1ull << 32)) {
-      g_batch_create_error = "instance dims must be >= 3 and a plane below 4 GiB";
+        nx * ny * 4 >= (1ull << 32) || nz > (1ull << 36) / (nx * ny)) {
+      g_batch_create_error = "instance dims must be >= 3, a plane below 4 GiB and at most 2^36 cells";
       return GD_ERR_ARG;
     }
   }
@@ -422,6 +422,11 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
     in.nz = dims[3 * i + 2];
     in.N = in.nx * in.ny * in.nz;
     bytes += 13 * align_up(in.N * sizeof(float), 256);
+    if (bytes > (size_t(1) << 50)) {  // far above any device; also keeps the sum from wrapping
+      g_batch_create_error = "batch larger than 1 PiB";
+      delete b;
+      return GD_ERR_NOMEM;
+    }
     int c = -1;
     for (size_t q = 0; q < b->cls.size(); ++q)
       if (b->cls[q].nx == in.nx && b->cls[q].ny == in.ny && b->cls[q].nz == in.nz) c = (int)q;

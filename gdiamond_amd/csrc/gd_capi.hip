@@ -329,7 +329,9 @@ int gd_create(gd_ctx** out, size_t nx, size_t ny, size_t nz, float dx) {
     g_create_error = "grid must be at least 3 cells on every axis";
     return GD_ERR_ARG;
   }
-  if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny > (1ull << 40)) {
+  // at most 2^40 cells in all, so that cell counts and byte sizes cannot wrap
+  if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny > (1ull << 40) ||
+      nz > (1ull << 40) / (nx * ny)) {
     g_create_error = "grid too large";
     return GD_ERR_ARG;
   }
@@ -493,7 +495,7 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (want_index) {
     if (c->storage == GD_STORAGE_F64) {
       if (c->mat_req == GD_MATERIAL_INDEXED)
-        return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or FP16 storage");
+        return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
     } else {
       std::vector<uint8_t> id;
       std::vector<float> table;
@@ -541,7 +543,7 @@ int gd_set_isotropic_indexed(gd_ctx* c, const uint8_t* id, const float* table, i
     if ((int)id[q] >= ntable) return c->fail(GD_ERR_ARG, "material id out of range");
   if (c->storage == GD_STORAGE_F64 || c->mat_req == GD_MATERIAL_DENSE) {
     if (c->storage == GD_STORAGE_F64 && c->mat_req == GD_MATERIAL_INDEXED)
-      return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or FP16 storage");
+      return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
     std::vector<float> cb(c->N);
     for (size_t q = 0; q < c->N; ++q) cb[q] = table[id[q]];
     const int req = c->mat_req;
@@ -832,7 +834,7 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     }
     zt = gd::ZTiling{t.blt, t.rows, t.waves, t.zchunk};
     if (c->storage == GD_STORAGE_F64)
-      return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel runs FP32 and FP16 storage only");
+      return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel runs FP32 and 16-bit storage only");
     if (!gd::ztiling_supported(zt, (gd::Storage)c->storage, c->mat_kind))
       return c->fail(GD_ERR_UNSUPPORTED, "unsupported tiling (blt 1..4, rows x waves in "
                                          "{2,4}x{4,8}, at least one owned row)");
@@ -888,6 +890,9 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
   if (d.ncomp) {
     const size_t nf = c->dft_f.size();
     const size_t need = 2 * nf * nsteps;
+    // one cos and one sin weight per frequency and step of this call
+    if (nf * nsteps > (size_t(1) << 27))
+      return c->fail(GD_ERR_ARG, "DFT weight table above 2 GiB: run fewer steps per call");
     if (need > c->dft_w_cap) {
       GD_HIP(c, hipStreamSynchronize(c->stream));
       if (c->dft_w) (void)hipFree(c->dft_w);

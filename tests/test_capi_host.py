@@ -108,6 +108,13 @@ def test_create_without_device_or_with_bad_sizes(gd):
     h = C.c_void_p()
     rc = gd.lib().gd_create(C.byref(h), 2, 10, 10, 1.0)
     assert rc == -1 and b"at least 3" in gd.lib().gd_last_error(None)
+    # sizes whose cell count (2^60) or byte size would wrap are refused before
+    # anything is allocated
+    rc = gd.lib().gd_create(C.byref(h), 1 << 20, 1 << 20, 1 << 20, 1.0)
+    assert rc == -1 and b"too large" in gd.lib().gd_last_error(None) and h.value is None
+    dims = (C.c_size_t * 3)(1 << 14, 1 << 14, 1 << 14)
+    rc = gd.lib().gd_batch_create(C.byref(h), 1, dims, 1.0)
+    assert rc == -1 and b"2^36 cells" in gd.lib().gd_batch_last_error(None)
     if gd.device_count() == 0:
         rc = gd.lib().gd_create(C.byref(h), 10, 10, 10, 1.0)
         assert rc == -6 and h.value is None
