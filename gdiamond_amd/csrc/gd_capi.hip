@@ -38,7 +38,7 @@ struct gd_ctx {
   int mat_req = GD_MATERIAL_AUTO;  // requested isotropic format
   gd::MatKind mat_kind = gd::kMatIso;
   void* coeff[12] = {nullptr};  // full material, compute type (float / double)
-  void* cb = nullptr;           // dense isotropic coefficient, storage type
+  void* cb = nullptr;           // dense isotropic coefficient, compute type (float / double)
   uint8_t* mat_id = nullptr;    // indexed isotropic: id per cell
   float* mat_table = nullptr;   // indexed isotropic: Cb per id (GD_MAX_MATERIALS floats)
   int ntable = 0;
@@ -487,46 +487,34 @@ int gd_get_material_format(const gd_ctx* c, int* format, int* ntable) {
 int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (!c) return GD_ERR_ARG;
   if (!cb) return c->fail(GD_ERR_ARG, "null cb array");
-  // AUTO: indexed for 16-bit storage (exact FP32 Cb instead of binary16),
-  // dense FP32 Cb otherwise (the same results; the dense tiled kernel is the
-  // faster one for FP32 fields)
-  const bool want_index = c->mat_req == GD_MATERIAL_INDEXED ||
-                          (c->mat_req == GD_MATERIAL_AUTO && is16(c->storage));
-  if (want_index) {
-    if (c->storage == GD_STORAGE_F64) {
-      if (c->mat_req == GD_MATERIAL_INDEXED)
-        return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
-    } else {
-      std::vector<uint8_t> id;
-      std::vector<float> table;
-      if (index_materials(cb, c->N, id, table))
-        return upload_indexed(c, id.data(), table.data(), (int)table.size(), db);
-      if (c->mat_req == GD_MATERIAL_INDEXED)
-        return c->fail(GD_ERR_UNSUPPORTED, "more than 256 distinct Cb values for the indexed "
-                                           "material");
-    }
+  // AUTO is the dense form: both forms give the same results, and the dense
+  // tiled kernel is the faster one for every field format (it reads 4 bytes
+  // of Cb per cell instead of 1, but needs no table lookup per level)
+  if (c->mat_req == GD_MATERIAL_INDEXED) {
+    if (c->storage == GD_STORAGE_F64)
+      return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
+    std::vector<uint8_t> id;
+    std::vector<float> table;
+    if (!index_materials(cb, c->N, id, table))
+      return c->fail(GD_ERR_UNSUPPORTED, "more than 256 distinct Cb values for the indexed "
+                                         "material");
+    return upload_indexed(c, id.data(), table.data(), (int)table.size(), db);
   }
   free_material(c);
-  hipError_t e = hipMalloc(&c->cb, c->N * c->esz);
+  // the coefficient is kept in the compute type: FP32 for FP32 and 16-bit
+  // fields (exact), FP64 for FP64 fields
+  const size_t ces = c->storage == GD_STORAGE_F64 ? sizeof(double) : sizeof(float);
+  hipError_t e = hipMalloc(&c->cb, c->N * ces);
   if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(cb)");
-  // the coefficient is stored in the field storage format (binary16 without
-  // scaling for 16-bit storage)
-  std::vector<unsigned char> buf(c->N * c->esz);
-  if (c->storage == GD_STORAGE_F32) {
-    std::memcpy(buf.data(), cb, c->N * sizeof(float));
-  } else if (c->storage == GD_STORAGE_F64) {
-    double* o = reinterpret_cast<double*>(buf.data());
+  if (c->storage == GD_STORAGE_F64) {
+    std::vector<double> buf(c->N);
     parallel_for(c->N, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) o[q] = (double)cb[q];
+      for (size_t q = b; q < e; ++q) buf[q] = (double)cb[q];
     });
+    GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * ces, hipMemcpyHostToDevice));
   } else {
-    const int st = c->storage;
-    uint16_t* o = reinterpret_cast<uint16_t*>(buf.data());
-    parallel_for(c->N, [&](size_t b, size_t e) {
-      for (size_t q = b; q < e; ++q) o[q] = enc16(st, cb[q]);
-    });
+    GD_HIP(c, hipMemcpy(c->cb, cb, c->N * ces, hipMemcpyHostToDevice));
   }
-  GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
   c->db = db;
   c->mat_kind = gd::kMatIso;
   c->have_mat = true;
@@ -800,7 +788,7 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
       if (c->f[k][a]) b += c->N * c->esz;
   for (int a = 0; a < 12; ++a)
     if (c->coeff[a]) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
-  if (c->cb) b += c->N * c->esz;
+  if (c->cb) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
   if (c->mat_id) b += c->N + GD_MAX_MATERIALS * sizeof(float);
   if (c->dft_snap) b += (size_t)gd::kMaxBlt * 6 * c->dft_plane * sizeof(float);
   for (auto& s : c->src) b += s.len * sizeof(float);

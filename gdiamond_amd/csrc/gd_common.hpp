@@ -43,7 +43,8 @@ struct Scales {
 
 // Material kinds.
 //   kMatIso:  isotropic lossless, Ca = Da = 1 (multiplication by 1 is exact and
-//             omitted), cb per cell in the storage format, db scalar.
+//             omitted), cb per cell in the compute type (FP32; FP64 for
+//             FP64 storage), db scalar.
 //   kMatFull: the reference's 12 arrays Cax Cbx Cay Cby Caz Cbz Dax Dbx Day Dby
 //             Daz Dbz in the compute type.
 //   kMatIdx:  isotropic lossless with a 1-byte material id per cell and an FP32

@@ -13,8 +13,9 @@
 //
 // Templated on the storage element S (float, double, or _Float16 / __bf16 with a
 // per-component power-of-two scale; arithmetic runs in Codec<S>::T) and on the
-// material kind (dense isotropic Cb, the reference's 12 arrays, or a 1-byte
-// material id into a small FP32 Cb table that stays in cache).
+// material kind (dense isotropic Cb in the compute type, the reference's 12
+// arrays, or a 1-byte material id into a small FP32 Cb table that stays in
+// cache).
 #include "gd_common.hpp"
 
 namespace gd {
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Scales sc, 
   const S* __restrict__ Hy = static_cast<const S*>(f.c[4]);
   const S* __restrict__ Hz = static_cast<const S*>(f.c[5]);
   const T* const* cf = reinterpret_cast<const T* const*>(m.c);
-  const S* __restrict__ cbp = static_cast<const S*>(m.cb);
+  const T* __restrict__ cbp = static_cast<const T*>(m.cb);
   const long long nx = g.nx, sxy = g.sxy;
   const T dx = (T)g.dx;
   long long idx = (long long)i + (long long)j * nx + (long long)k0 * sxy;
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(kBx* kBy) void plain_e_kernel(Fields f, Scales sc, 
           if constexpr (kMat == kMatIdx)
             b = (T)m.table[m.id[idx]];
           else
-            b = C::dec(cbp[idx], 1.0f);
+            b = cbp[idx];
           ex = ex + b * cx;
           ey = ey + b * cy;
           ez = ez + b * cz;

@@ -240,9 +240,11 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
 
   // Level-0 values of the next stream position, loaded one iteration ahead so
   // their HBM latency overlaps the level updates of the current position
-  // (kept in the storage format; decoded when consumed). Planes >= nz are
-  // clamped to nz-1: they only feed cells outside the cone.
-  using MT = typename std::conditional<kIdx, uint8_t, S>::type;
+  // (fields kept in the storage format and decoded when consumed; the dense
+  // Cb is FP32 whatever the field format). Planes >= nz are clamped to nz-1:
+  // they only feed cells outside the cone.
+  using MT = typename std::conditional<kIdx, uint8_t, float>::type;
+  constexpr unsigned kCbMul = 4 / ES;  // FP32 Cb offsets from field offsets
   S pe[R][3], ph[R][3], phmz, phmx;
   MT pcb[R];
   auto prefetch = [&](int pp) {
@@ -262,7 +264,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
       for (int r = 0; r < R; ++r) pcb[r] = ldp<uint8_t>(a.m.id, pz >> kLogES, off[r] >> kLogES);
     } else if constexpr (!kFull) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) pcb[r] = ldp<S>(a.m.cb, pz, off[r]);
+      for (int r = 0; r < R; ++r) pcb[r] = ldp<float>(a.m.cb, pz * kCbMul, off[r] * kCbMul);
     }
   };
   prefetch(p_begin);
@@ -288,7 +290,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
 #pragma unroll
         for (int r = 0; r < R; ++r) cbq[b][r] = cbq[b - 1][r];
 #pragma unroll
-      for (int r = 0; r < R; ++r) cbq[0][r] = C::dec(pcb[r], 1.0f);
+      for (int r = 0; r < R; ++r) cbq[0][r] = pcb[r];
     }
     prefetch(p + 1);
 

@@ -122,22 +122,21 @@ int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
 /* Device formats of the isotropic material (gd_set_material_format), and what
  * gd_get_material_format reports. */
 enum {
-  GD_MATERIAL_AUTO = 0,    /* 16-bit storage: INDEXED when Cb has at most GD_MAX_MATERIALS
-                              distinct values, else DENSE; other storage formats: DENSE */
-  GD_MATERIAL_DENSE = 1,   /* one Cb per cell in the field storage format */
+  GD_MATERIAL_AUTO = 0,    /* DENSE for gd_set_isotropic(_eps), INDEXED for
+                              gd_set_isotropic_indexed */
+  GD_MATERIAL_DENSE = 1,   /* one Cb per cell in FP32 (FP64 with GD_STORAGE_F64) */
   GD_MATERIAL_INDEXED = 2, /* one 1-byte material id per cell + an FP32 table of Cb values */
   GD_MATERIAL_FULL = 3     /* reported only: the 12-array form of gd_set_coefficients */
 };
 #define GD_MAX_MATERIALS 256
 
 /* Choose how the isotropic Cb is held on the device; must precede the
- * material (default GD_MATERIAL_AUTO). INDEXED keeps Cb exact in FP32 for
- * every field storage format and costs 1 B per cell-update instead of 4 (FP32)
- * or 2 (16-bit); with 16-bit storage the DENSE form stores Cb as binary16 or
- * bfloat16, whose rounding changes the medium (relative Cb error up to 2^-11,
- * 2^-8). INDEXED is not
- * available with GD_STORAGE_F64. No counterpart in the reference, which
- * streams 12 FP32 coefficient arrays (gdiamond_gpu.cuh:871-887). */
+ * material (default GD_MATERIAL_AUTO). Both forms keep Cb exact and give
+ * bit-identical results: DENSE reads 4 B per cell-update and is the faster
+ * tiled kernel for every field format; INDEXED reads 1 B and holds a quarter
+ * of the device memory. INDEXED is not available with GD_STORAGE_F64. No
+ * counterpart in the reference, which streams 12 FP32 coefficient arrays
+ * (gdiamond_gpu.cuh:871-887). */
 int gd_set_material_format(gd_ctx* ctx, int format);
 /* Format in use (DENSE, INDEXED or FULL; -1 before any material) and the
  * number of table entries of the INDEXED form (0 otherwise). */

@@ -71,8 +71,7 @@ int or_run_f64(int64_t nx, int64_t ny, int64_t nz, double dx,
 
 /* FP16-storage twin: every component c is stored as binary16 of
  * value * 2^-scale_exp[c]; values are decoded exactly to float, computed in
- * float with the operand order of or_run_f32 (isotropic Cb given in float:
- * exact table values, or decoded binary16 for the dense 16-bit coefficient)
+ * float with the operand order of or_run_f32 (isotropic Cb given in float)
  * and rounded to binary16 (nearest even) when they are stored:
  *   block == 0: after every half step (E, then H), as an untiled update that
  *               writes E and H to memory each step does; the DFT sees the

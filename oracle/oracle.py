@@ -235,8 +235,7 @@ def _dft_struct(dft, shape, keep):
 def run_f16(shape, fields16, scale_exp, cb, db, dx=1.0, sources=(), n0=0, nsteps=1,
             dft=None, block=0, threads=1, fmt="f16"):
     """FP16-storage twin: fields16 are 6 uint16 (binary16 bit) arrays shaped
-    (nz, ny, nx), advanced in place; cb the float32 Cb array (exact values, or
-    the decoded binary16 coefficient of the dense 16-bit form). ``block`` 0
+    (nz, ny, nx), advanced in place; cb the float32 Cb array. ``block`` 0
     rounds after every half step (untiled update), block B after every B steps
     (temporally blocked update with B levels per launch); ``threads`` host
     threads run the block mode, with results independent of the count.

@@ -121,9 +121,7 @@ def _oracle_run(oracle, case, blt):
                 (oracle.encode_f16, oracle.decode_f16))
     init = case["init"] or [np.zeros((nz, ny, nx), np.float32)] * 6
     bits = [enc(init[c], se[c]) for c in range(6)]
-    cb = case["cb"]
-    if case["mat"] == "dense":  # 16-bit coefficient of the dense 16-bit form
-        cb = dec(enc(cb, 0))
+    cb = case["cb"]  # exact FP32 Cb in both material forms
     n0 = 0
     for n in case["chunks"]:
         d["re"], d["im"] = oracle.run_f16(shape, bits, se, cb, case["db"], sources=case["sources"],

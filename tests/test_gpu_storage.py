@@ -41,24 +41,15 @@ def _codec(oracle, fmt):
     return oracle.encode_f16, oracle.decode_f16
 
 
-def _cb_effective(oracle, cb, mat, fmt="f16"):
-    """Cb the kernels compute with: exact for the indexed material, the 16-bit
-    value for the dense material of 16-bit storage."""
-    enc, dec = _codec(oracle, fmt)
-    if mat == "dense":
-        return dec(enc(cb, 0))
-    return cb
-
-
-def _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, chunks=None, mat="indexed",
-                block=0, fmt="f16"):
+def _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, chunks=None, block=0,
+                fmt="f16"):
+    """Oracle twin of a 16-bit run; Cb is exact FP32 in both material forms."""
     enc, dec = _codec(oracle, fmt)
     f16 = [enc(init[c], scale_exp[c]) for c in range(6)]
-    cb16 = _cb_effective(oracle, cb, mat, fmt)
     dft = dict(dft)
     n0 = 0
     for n in (chunks or [T]):
-        dft["re"], dft["im"] = oracle.run_f16(shape, f16, scale_exp, cb16, db, sources=src,
+        dft["re"], dft["im"] = oracle.run_f16(shape, f16, scale_exp, cb, db, sources=src,
                                               n0=n0, nsteps=n, dft=dft, block=block, fmt=fmt)
         n0 += n
     re, im = dft["re"], dft["im"]
@@ -110,7 +101,7 @@ def test_f16_plain_matches_oracle(gd, oracle, scale_exp, mat):
     T = 40
     cb, db, src, dft = _f16_problem(gd, shape, T)
     init = random_fields(shape, 5, scale=0.05, boundary_zero=True)
-    want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, mat=mat)
+    want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft)
     say("f16 plain", shape, scale_exp, mat)
     got = _gpu(gd, shape, gd.F16, scale_exp, cb, db, src, T, dft, init=init, variant=gd.PLAIN,
                mat=mat)
@@ -125,7 +116,7 @@ def test_f16_tiled_matches_oracle(gd, oracle, tiling, mat):
     scale_exp = SCALES[1]
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=3)
     init = random_fields(shape, 9, scale=0.05, boundary_zero=True)
-    want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, mat=mat,
+    want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft,
                        block=tiling["blt"])
     say("f16 tiled", shape, tiling, mat)
     got = _gpu(gd, shape, gd.F16, scale_exp, cb, db, src, T, dft, init=init, variant=gd.TILED,
@@ -157,10 +148,15 @@ def test_material_formats(gd, oracle):
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=12)
     distinct = np.unique(cb.view(np.uint32)).size
     assert 2 <= distinct <= 256
-    # AUTO picks the indexed form for a sphere medium with FP16 fields, with one
-    # entry per value, and the dense form with FP32 fields
+    # AUTO picks the dense form for every field format; INDEXED holds one table
+    # entry per distinct value
     with gd.Context(*shape) as ctx:
         ctx.set_storage(gd.F16, [0] * 6)
+        ctx.set_isotropic(cb, db)
+        assert ctx.get_material_format() == (gd.MAT_DENSE, 0)
+    with gd.Context(*shape) as ctx:
+        ctx.set_storage(gd.BF16, [0] * 6)
+        ctx.set_material_format(gd.MAT_INDEXED)
         ctx.set_isotropic(cb, db)
         assert ctx.get_material_format() == (gd.MAT_INDEXED, distinct)
     with gd.Context(*shape) as ctx:
@@ -215,7 +211,7 @@ def test_f16_runs_continue(gd, oracle):
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=8)
     for variant, tiling in ((gd.PLAIN, None), (gd.TILED, dict(blt=3, rows=2, waves=8, zchunk=7))):
         want = _oracle_f16(oracle, shape, [np.zeros((29, 37, 52), np.float32)] * 6, scale_exp, cb,
-                           db, src, T, dft, chunks=chunks, mat="indexed",
+                           db, src, T, dft, chunks=chunks,
                            block=0 if tiling is None else tiling["blt"])
         say("f16 continue", variant, tiling)
         got = _gpu(gd, shape, gd.F16, scale_exp, cb, db, src, T, dft, variant=variant,
@@ -238,7 +234,7 @@ def test_bf16_matches_oracle(gd, oracle, tiling, mat):
     init = random_fields(shape, 4, scale=0.05, boundary_zero=True)
     chunks = [9, 12]
     want = _oracle_f16(oracle, shape, init, scale_exp, cb, db, src, T, dft, chunks=chunks,
-                       mat=mat, block=0 if tiling is None else tiling["blt"], fmt="bf16")
+                       block=0 if tiling is None else tiling["blt"], fmt="bf16")
     say("bf16", shape, tiling, mat)
     got = _gpu(gd, shape, gd.BF16, scale_exp, cb, db, src, T, dft, init=init,
                variant=gd.PLAIN if tiling is None else gd.TILED, tiling=tiling, mat=mat,
@@ -291,8 +287,7 @@ def test_storage_error_vs_f64(gd):
     """FP32 and FP16 storage against the FP64 run of the same problem, by the
     relative DFT-spectrum error ||X - X_ref||_2 / ||X_ref||_2 per component and
     frequency. With the indexed material Cb is exact and only the per-step
-    binary16 rounding of the fields remains; the dense binary16 Cb changes the
-    medium itself (relative Cb error up to 2^-11, a phase-velocity error)."""
+    binary16 rounding of the fields remains."""
     shape = (96, 80, 72)
     T = 300
     eps, cb, db = iso_problem(gd, shape, seed=5, nspheres=30, rmin=3.0, rmax=9.0)
@@ -306,7 +301,6 @@ def test_storage_error_vs_f64(gd):
     for name, st, se, mat, var in (("f32", gd.F32, None, "indexed", gd.TILED),
                                    ("f16", gd.F16, [-2] * 6, "indexed", gd.TILED),
                                    ("f16-plain", gd.F16, [-2] * 6, "indexed", gd.PLAIN),
-                                   ("f16-dense-cb", gd.F16, [-2] * 6, "dense", gd.TILED),
                                    ("bf16", gd.BF16, [0] * 6, "indexed", gd.TILED)):
         _, (re, im) = _gpu(gd, shape, st, se, cb, db, src, T, dft, variant=var, mat=mat)
         err[name] = spectral_error(re, im, ref_re, ref_im)
@@ -316,8 +310,7 @@ def test_storage_error_vs_f64(gd):
     # (PLAIN) lowers the error
     assert err["f16"].max() < 1e-3
     assert err["f16-plain"].max() < 3e-3
-    assert err["f16-dense-cb"].max() < 3e-2
-    assert err["f16-dense-cb"].max() > err["f16-plain"].max() > err["f16"].max() > err["f32"].max()
+    assert err["f16-plain"].max() > err["f16"].max() > err["f32"].max()
     # bfloat16 keeps 8 significand bits against binary16's 11: about 2^3 times
     # the per-rounding error, with the same rounding schedule
     assert 2 * err["f16"].max() < err["bf16"].max() < 16 * err["f16"].max()
