@@ -245,6 +245,8 @@ struct ZArgs {
   int run_step0;  // DFT weight column of level 1
   int ntx, nty, ntz;
   int zc;         // planes owned per workgroup
+  int scw;        // tile order: 0 = x fastest along whole rows; w > 0 = columns
+                  // of w tiles, x fastest within a column, when ntx > w
 };
 
 // Tile decomposition of an nx*ny*nz problem under a tiling.

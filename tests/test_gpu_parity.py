@@ -125,6 +125,29 @@ def test_iso_sphere_medium_plain_and_tiled(gpu, oracle, shape):
         assert_fields_equal(got, want, f"tiled {t} {shape}")
 
 
+@pytest.mark.parametrize("shape", [(1300, 30, 9), (616, 50, 12)])
+def test_wide_grids_column_tile_order(gpu, oracle, shape):
+    """Rows of more than 10 tiles: FP32 launches walk the tiles in columns
+    10 tiles wide (the last one narrower: 24 tiles = 10+10+4, 11 = 10+1 at
+    B=4). The order changes which workgroups run together, never a value."""
+    eps, cb, db = iso_problem(gpu, shape, seed=21, nspheres=8, rmin=2, rmax=6)
+    T = 9
+    src = gpu.gaussian_pulse_table(T, 4.0, 2.0, 0.07)
+    i, j, k = centre(shape)
+    setup = dict(cb=cb, db=db, fields=random_fields(shape, 31, scale=0.05),
+                 sources=[(gpu.EZ, i, j, k, src), (gpu.HY, shape[0] - 40, 3, 2, src * 0.5)],
+                 dft=dict(comps=[0, 2, 4], axis=0, index=shape[0] - 30,
+                          f_cps=np.array([0.05, 0.09])))
+    want, (wre, wim) = oracle_run(oracle, shape, setup, T)
+    for t in [None] + TILINGS:
+        say(f"[wide] {shape} tiling {t}")
+        got, dft = run_variant(gpu, shape, setup, T, gpu.TILED, t)
+        assert_fields_equal(got, want, f"wide {shape} {t}")
+        for n, c in enumerate(setup["dft"]["comps"]):
+            assert bits_equal(dft[c][0], wre[n]) and bits_equal(dft[c][1], wim[n]), \
+                f"wide {shape} {t} DFT comp {c}"
+
+
 def test_random_initial_fields_full_material(gpu, oracle):
     """Random fields (boundary included) + random full coefficient arrays."""
     shape = (41, 38, 27)
