@@ -87,6 +87,18 @@ def cpu_baseline(gd, n, eps, dt, src, steps, threads):
     return n ** 3 * steps / dt_s / 1e9
 
 
+def host_cpu():
+    """CPU model name of the host, from /proc/cpuinfo (None where absent)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 ERROR_BOUND = 1e-3  # relative DFT-spectrum error of a form counted as "at fixed error"
 
 
@@ -566,7 +578,8 @@ def main():
                                "cores": threads, "kind": "port",
                                "sample": f"same {n}^3 workload, steps 1..{args.cpu_steps} of {T} "
                                          f"(after one untimed first-touch step), OpenMP "
-                                         f"collapse(2) port of update_FDTD_omp"}
+                                         f"collapse(2) port of update_FDTD_omp",
+                               "host": host_cpu(), "nproc": os.cpu_count()}
     print(json.dumps(out), flush=True)
 
 
