@@ -580,6 +580,11 @@ def main():
                                          f"(after one untimed first-touch step), OpenMP "
                                          f"collapse(2) port of update_FDTD_omp",
                                "host": host_cpu(), "nproc": os.cpu_count()}
+        seq_steps = 4
+        seq = cpu_baseline(gd, n, eps, dt, src, seq_steps, 1)
+        out["cpu_baseline"]["single_thread"] = {
+            "value": round(seq, 4), "unit": "Gcell-updates/s",
+            "sample": f"steps 1..{seq_steps}, one thread (the update_FDTD_seq loop)"}
     print(json.dumps(out), flush=True)
 
 
