@@ -84,6 +84,8 @@ _SIGS = {
     "gd_dft_disable": (C.c_int, [C.c_void_p]),
     "gd_run": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.POINTER(Tiling)]),
     "gd_plan_tiling": (C.c_int, [C.c_void_p, C.POINTER(Tiling)]),
+    "gd_plan_tiling_for": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_int,
+                                     C.POINTER(Tiling)]),
     "gd_last_run_stats": (C.c_int, [C.c_void_p, _DP, _DP, C.POINTER(C.c_size_t)]),
     "gd_set_launch_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gd_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
@@ -164,6 +166,16 @@ def device_count() -> int:
     n = C.c_int(0)
     lib().gd_device_count(C.byref(n))
     return n.value
+
+
+def plan_tiling_for(nx, ny, nz, storage=F32):
+    """The planner's TILED tiling for a grid and storage format (no device
+    needed)."""
+    t = Tiling()
+    rc = lib().gd_plan_tiling_for(int(nx), int(ny), int(nz), int(storage), C.byref(t))
+    if rc != 0:
+        raise GdError(rc, "bad grid or storage format")
+    return t.as_dict()
 
 
 def _f32(a, n=None):

@@ -281,32 +281,38 @@ hipEvent_t pool_event(gd_ctx* c, size_t i) {
   return c->ev_pool[i];
 }
 
-// Tiling planner: B = 4 time levels, 4 rows x 8 waves (64 x 32 input tile,
-// 56 x 24 owned), the fastest measured shape at 512^3 on MI355X. The z chunk
-// is at most 64 owned planes and shrinks (down to 16) until the launch has
-// ~1024 workgroups (4 per CU on 256 CUs).
-gd_tiling plan(const gd_ctx* c) {
+// Tiling planner: 4 rows x 8 waves (a 64 x 32 input tile), the fastest
+// measured shape on MI355X, with B levels per launch. The z chunk is at most
+// 64 owned planes and shrinks (down to 16) until the launch has ~1024
+// workgroups (4 per CU on 256 CUs).
+gd_tiling plan_blt(size_t nx, size_t ny, size_t nz, int storage, int blt) {
   gd_tiling t;
-  // Four levels per launch for 4- and 8-byte fields. With binary16 fields the
-  // HBM share of a launch is half as large and three levels run faster
-  // (227 vs 185 Gcell/s at 512^3: less overlap, no register spills).
-  t.blt = is16(c->storage) ? 3 : 4;
+  t.blt = blt;
   t.rows = 4;
   t.waves = 8;
   const size_t ox = 64 - 2 * t.blt, oy = (size_t)(t.rows * t.waves - 2 * t.blt);
-  const size_t txy = ((c->nx + ox - 1) / ox) * ((c->ny + oy - 1) / oy);
-  size_t zc = c->nz * txy / 1024;
+  const size_t txy = ((nx + ox - 1) / ox) * ((ny + oy - 1) / oy);
+  size_t zc = nz * txy / 1024;
   if (zc < 16) zc = 16;
   if (zc > 64) zc = 64;
   // binary16 launches are held by their arithmetic more than by HBM: a 128-plane
   // chunk halves the recomputed z halo, worth it while the grid still gives
   // at least 16 workgroups per CU (1024^3: 258 vs 251 Gcell/s; the FP32 form
   // measured slower with longer chunks)
-  if (is16(c->storage) && txy * ((c->nz + 127) / 128) >= 4096) zc = 128;
-  if (zc > c->nz) zc = c->nz;
+  if (is16(storage) && txy * ((nz + 127) / 128) >= 4096) zc = 128;
+  if (zc > nz) zc = nz;
   t.zchunk = (int)zc;
   return t;
 }
+
+gd_tiling plan_dims(size_t nx, size_t ny, size_t nz, int storage) {
+  // Four levels per launch for 4- and 8-byte fields. With 16-bit fields the
+  // HBM share of a launch is half as large and three levels run faster
+  // (227 vs 185 Gcell/s at 512^3: less overlap, no register spills).
+  return plan_blt(nx, ny, nz, storage, is16(storage) ? 3 : 4);
+}
+
+gd_tiling plan(const gd_ctx* c) { return plan_dims(c->nx, c->ny, c->nz, c->storage); }
 
 }  // namespace
 
@@ -763,6 +769,14 @@ int gd_dft_get(gd_ctx* c, int comp, double* re, double* im) {
 int gd_plan_tiling(const gd_ctx* c, gd_tiling* out) {
   if (!c || !out) return GD_ERR_ARG;
   *out = plan(c);
+  return GD_OK;
+}
+
+int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* out) {
+  if (!out || nx < 3 || ny < 3 || nz < 3) return GD_ERR_ARG;
+  if (storage != GD_STORAGE_F32 && storage != GD_STORAGE_F64 && !is16(storage))
+    return GD_ERR_ARG;
+  *out = plan_dims(nx, ny, nz, storage);
   return GD_OK;
 }
 

@@ -63,7 +63,7 @@ enum { GD_EX = 0, GD_EY = 1, GD_EZ = 2, GD_HX = 3, GD_HY = 4, GD_HZ = 5 };
 enum { GD_VARIANT_PLAIN = 0, GD_VARIANT_TILED = 1 };
 
 /* Tiling of GD_VARIANT_TILED. Zero fields select the planner's choice.
- *  blt    : time steps per launch (1..GD_MAX_BLT)
+ *  blt    : time steps per launch (1..4)
  *  rows   : y rows held by each thread
  *  waves  : wave64 wavefronts per workgroup (tile height = rows*waves)
  *  zchunk : z planes owned by one workgroup */
@@ -208,6 +208,13 @@ int gd_dft_disable(gd_ctx* ctx);
 int gd_run(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
 /* The tiling the planner would use for GD_VARIANT_TILED on this context. */
 int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
+/* The planner's tiling for a grid and storage format, without a context or
+ * a device.
+ * Planner rule: 4 rows x 8 waves; 3 levels for 16-bit fields, 4 otherwise;
+ * z chunks of 16..64 planes sized for ~1024 workgroups (128 for 16-bit
+ * fields on grids that still give 4096). No counterpart in the reference,
+ * whose tile sizes are compile-time constants (kernels_mm_ver2.cuh:8-32). */
+int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* out);
 
 /* Timing of the last gd_run, from HIP events on the context's stream:
  * total_ms spans the whole stepping loop; with per-launch timing enabled
