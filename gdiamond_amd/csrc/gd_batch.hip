@@ -753,7 +753,8 @@ int gd_batch_run(gd_batch* b, size_t nsteps, int mode) {
           gd::ZTiling t{blt, g.rows, g.waves, b->cls[in.cls].t.zchunk};
           gd::ZArgs a = make_args(b, i, par, t);
           BH(b, gd::launch_zstream(t, gd::kF32, a.src, a.dst, a.sc, a.m, gd::kMatIso, a.g, a.s,
-                                   a.d, (int)(b->step + (size_t)k * g.blt), 0, b->stream));
+                                   a.d, (int)(b->step + (size_t)k * g.blt), 0,
+                                   gd::ZPart{gd::kZFull, nullptr, 0}, b->stream));
           par ^= 1;
           ++nl;
         }

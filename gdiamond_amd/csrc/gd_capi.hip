@@ -68,6 +68,18 @@ struct gd_ctx {
   float* dft_snap = nullptr;  // TILED: [level][component][plane] plane values
   std::vector<double> dft_w_host;
 
+  // TILED launches split in two: tiles with sources or monitored cells run
+  // the full kernel on `side`, the others the lean kernel on `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  struct Split {
+    int blt = 0;  // tiling the classification is for (0: none yet in this run)
+    std::vector<int> ids;  // tile ids: the extra tiles, then the lean ones
+    size_t nextra = 0;
+    int* d_ids = nullptr;
+    size_t cap = 0;  // ids the device buffer holds
+  } split[2];
+
   bool launch_timing = false;
   std::vector<hipEvent_t> ev_pool;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -390,6 +402,14 @@ int gd_destroy(gd_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (int k = 0; k < 2; ++k)
     if (c->cal[k]) (void)hipFree(c->cal[k]);
+  for (auto& sp : c->split)
+    if (sp.d_ids) (void)hipFree(sp.d_ids);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GD_OK;
@@ -948,10 +968,55 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
     }
   } else {
+    // With the planner's 4 x 8 tile a launch is split: the few tiles that hold
+    // a source cell in their window or own monitored cells run the full
+    // kernel on the side stream while every other tile runs the lean kernel
+    // (no source or DFT code in its loop) on the context's stream.
+    const bool lean = gd::zstream_lean_supported(zt);
+    const bool extras = s.n > 0 || d.ncomp > 0;
+    if (lean && extras && !c->side) {
+      GD_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+      GD_HIP(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+      GD_HIP(c, hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+    }
+    for (auto& sp : c->split) sp.blt = 0;
+    // classifies the tiles of tiling t into sp and uploads both id lists on
+    // the context's stream (ahead of the launches that read them)
+    std::vector<int> lean_ids;
+    auto classify = [&](const gd::ZTiling& t, gd_ctx::Split& sp) -> int {
+      gd::zstream_classify(t, st, g, c->src_desc_host.data(), s.n, d, sp.ids, lean_ids);
+      sp.nextra = sp.ids.size();
+      sp.ids.insert(sp.ids.end(), lean_ids.begin(), lean_ids.end());
+      const size_t n = sp.ids.size();
+      if (n > sp.cap) {
+        if (sp.d_ids) (void)hipFree(sp.d_ids);
+        sp.d_ids = nullptr;
+        sp.cap = 0;
+        hipError_t e = hipMalloc(&sp.d_ids, n * sizeof(int));
+        if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(tile lists)");
+        sp.cap = n;
+      }
+      GD_HIP(c, hipMemcpyAsync(sp.d_ids, sp.ids.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                               c->stream));
+      sp.blt = t.blt;
+      return GD_OK;
+    };
     size_t r = 0;
     while (r < nsteps) {
       gd::ZTiling t = zt;
       t.blt = (int)std::min<size_t>((size_t)zt.blt, nsteps - r);
+      auto launch = [&](const gd::ZPart& part, hipStream_t strm) {
+        return gd::launch_zstream(t, st, fields_of(c, c->cur), fields_of(c, 1 - c->cur), sc, m,
+                                  c->mat_kind, g, s, d, (int)(c->step + r), (int)r, part, strm);
+      };
+      gd_ctx::Split* sp = nullptr;
+      if (lean && extras) {
+        sp = &c->split[t.blt == zt.blt ? 0 : 1];
+        if (sp->blt != t.blt) {
+          const int rc = classify(t, *sp);
+          if (rc != GD_OK) return rc;
+        }
+      }
       hipEvent_t a = nullptr, b = nullptr;
       if (c->launch_timing) {
         a = pool_event(c, nev++);
@@ -959,8 +1024,20 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
         if (!a || !b) return c->fail(GD_ERR_HIP, "event creation failed");
         GD_HIP(c, hipEventRecord(a, c->stream));
       }
-      GD_HIP(c, gd::launch_zstream(t, st, fields_of(c, c->cur), fields_of(c, 1 - c->cur), sc, m,
-                                   c->mat_kind, g, s, d, (int)(c->step + r), (int)r, c->stream));
+      const int ne = sp ? (int)sp->nextra : 0;
+      const int nl = sp ? (int)(sp->ids.size() - sp->nextra) : 0;
+      if (!lean || (sp && nl == 0)) {
+        GD_HIP(c, launch(gd::ZPart{gd::kZFull, nullptr, 0}, c->stream));
+      } else if (!sp || ne == 0) {
+        GD_HIP(c, launch(gd::ZPart{gd::kZLean, nullptr, 0}, c->stream));
+      } else {
+        GD_HIP(c, hipEventRecord(c->fork_ev, c->stream));
+        GD_HIP(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
+        GD_HIP(c, launch(gd::ZPart{gd::kZFull, sp->d_ids, ne}, c->side));
+        GD_HIP(c, launch(gd::ZPart{gd::kZLean, sp->d_ids + ne, nl}, c->stream));
+        GD_HIP(c, hipEventRecord(c->join_ev, c->side));
+        GD_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+      }
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
       if (d.ncomp) GD_HIP(c, gd::launch_dft_snapshot(d, t.blt, (int)r, c->stream));
       c->cur = 1 - c->cur;

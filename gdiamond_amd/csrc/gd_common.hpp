@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace gd {
 
 constexpr int kMaxSources = 8;
@@ -284,10 +286,29 @@ int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device);
 hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* flip, int ngroups,
                                 unsigned* counters, int ncounters, hipStream_t stream);
 bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk);
+// Kernel forms of a TILED launch: kZFull runs tiles with the point-source and
+// DFT code, kZLean without it (for tiles that need neither). `list` (device,
+// nlist tile ids tx + ntx*(ty + nty*tz)) restricts the launch to those
+// tiles; null runs every tile. The two lists of one classification
+// (zstream_classify) together cover every tile once.
+enum ZForm { kZFull = 0, kZLean = 1 };
+struct ZPart {
+  int form;
+  const int* list;
+  int nlist;
+};
+// The lean form exists for the 4 x 8 tile (the planner's).
+bool zstream_lean_supported(const ZTiling& t);
+// Host: splits the tiles of tiling t into those that need the source or DFT
+// code (a source cell in the tile's input window and streamed z range, or
+// owned cells on the monitored plane) and the rest, each list in the
+// kernels' tile order.
+void zstream_classify(const ZTiling& t, Storage st, const Grid& g, const SourceDesc* src,
+                      int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean);
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,
                           const Sources& s, const Dft& d, int n0, int run_step0,
-                          hipStream_t stream);
+                          const ZPart& part, hipStream_t stream);
 
 hipError_t launch_copy(const float* src, float* dst, size_t n, hipStream_t stream);
 

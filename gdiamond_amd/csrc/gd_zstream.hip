@@ -35,6 +35,7 @@
 // DFT snapshots hold the FP32 value of every level. Every workgroup that
 // computes a cell at some level does so from the same decoded inputs with the
 // same operations, so overlapping tiles agree bit for bit.
+#include <algorithm>
 #include <type_traits>
 
 #include "gd_common.hpp"
@@ -113,11 +114,38 @@ constexpr int zstream_waves_per_simd() {
   return R <= 2 ? 4 : 2;
 }
 
-// One workgroup's work: tile `tile` (x fastest, then y, then z chunk; or in
-// columns of a.scw tiles, see ZArgs) of the problem described by `a`, streamed
-// through B time levels.
-template <int B, int R, int W, int kMat, class S>
-__device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
+// Tile coordinates of launch position `tile`: x fastest, then y, then z
+// chunk, or in columns of a.scw tiles (see ZArgs).
+__host__ __device__ __forceinline__ void tile_coords(const ZArgs& a, const int tile, int& tx,
+                                                     int& ty, int& tz) {
+  if (a.scw <= 0 || a.ntx <= a.scw) {
+    tx = tile % a.ntx;
+    const int tyz = tile / a.ntx;
+    ty = tyz % a.nty;
+    tz = tyz / a.nty;
+  } else {
+    // super-columns of scw tiles (the last one narrower), walked x fastest,
+    // then y, then the next column
+    const int per_z = a.ntx * a.nty;
+    tz = tile / per_z;
+    const int rem = tile - tz * per_z;
+    const int col = rem / (a.scw * a.nty);
+    const int within = rem - col * a.scw * a.nty;
+    const int left = a.ntx - col * a.scw;
+    const int width = a.scw < left ? a.scw : left;
+    ty = within / width;
+    tx = col * a.scw + (within - ty * width);
+  }
+}
+
+// One workgroup's work: tile (tx, ty, tz) of the problem described by `a`,
+// streamed through B time levels. kX = false compiles out the point sources
+// and the DFT snapshots (tiles without a source cell in their window and
+// without owned monitored cells): their state no longer competes for
+// registers in the stream loop.
+template <int B, int R, int W, int kMat, class S, bool kX>
+__device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const int ty,
+                                             const int tz) {
   using C = Codec<S>;
   constexpr bool kFull = kMat == kMatFull;
   constexpr bool kIdx = kMat == kMatIdx;
@@ -136,24 +164,6 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  int tx, ty, tz;
-  if (a.scw <= 0 || a.ntx <= a.scw) {
-    tx = tile % a.ntx;
-    const int tyz = tile / a.ntx;
-    ty = tyz % a.nty;
-    tz = tyz / a.nty;
-  } else {
-    // super-columns of scw tiles (the last one narrower), walked x fastest,
-    // then y, then the next column
-    const int per_z = a.ntx * a.nty;
-    tz = tile / per_z;
-    const int rem = tile - tz * per_z;
-    const int col = rem / (a.scw * a.nty);
-    const int within = rem - col * a.scw * a.nty;
-    const int width = min(a.scw, a.ntx - col * a.scw);
-    ty = within / width;
-    tx = col * a.scw + (within - ty * width);
-  }
 
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const size_t plane_bytes = (size_t)a.g.sxy * ES;
@@ -182,7 +192,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
   }
   const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * ES;
   // wave-uniform: does any lane of this wave hold a source column?
-  const bool wave_src = __builtin_amdgcn_ballot_w64(any_src) != 0;
+  const bool wave_src = kX && __builtin_amdgcn_ballot_w64(any_src) != 0;
 
   const int z_lo = tz * a.zc;
   const int z_hi = min(nz, z_lo + a.zc);
@@ -204,7 +214,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
     dft_any = dft_any || (dft_on && own[r] && on);
   }
   // wave-uniform: does this wave own any cell of the monitored plane?
-  const bool wave_dft = __builtin_amdgcn_ballot_w64(dft_any) != 0;
+  const bool wave_dft = kX && __builtin_amdgcn_ballot_w64(dft_any) != 0;
   // per-row monitor predicate and plane index base, evaluated only by waves
   // that own monitored cells
   auto dft_row = [&](int r) {
@@ -482,9 +492,40 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tile) {
   }
 }
 
+// Tile of this workgroup: launch position xcd_tile(blockIdx) in tile order,
+// or, with a list, the tile whose id (tx + ntx*(ty + nty*tz)) the list holds
+// at that position. Lists are in tile order, so each XCD still takes a
+// contiguous run of neighbouring tiles.
+__device__ __forceinline__ void launch_tile(const ZArgs& a, const int* list, int& tx, int& ty,
+                                            int& tz) {
+  const int pos = xcd_tile(blockIdx.x, gridDim.x);
+  if (list) {
+    const int id = list[pos];
+    tx = id % a.ntx;
+    ty = (id / a.ntx) % a.nty;
+    tz = id / (a.ntx * a.nty);
+  } else {
+    tile_coords(a, pos, tx, ty, tz);
+  }
+}
+
+// Every tile of the launch (list null) or the listed tiles, with the source
+// and DFT code.
 template <int B, int R, int W, int kMat, class S>
-__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(ZArgs a) {
-  zstream_tile<B, R, W, kMat, S>(a, xcd_tile(blockIdx.x, gridDim.x));
+__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_kernel(
+    ZArgs a, const int* __restrict__ list) {
+  int tx, ty, tz;
+  launch_tile(a, list, tx, ty, tz);
+  zstream_tile<B, R, W, kMat, S, true>(a, tx, ty, tz);
+}
+
+// The same without the source and DFT code, for tiles that need neither.
+template <int B, int R, int W, int kMat, class S>
+__global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_lean_kernel(
+    ZArgs a, const int* __restrict__ list) {
+  int tx, ty, tz;
+  launch_tile(a, list, tx, ty, tz);
+  zstream_tile<B, R, W, kMat, S, false>(a, tx, ty, tz);
 }
 
 // Copies a wave-uniform object from global memory into registers through
@@ -522,7 +563,9 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_b
     ZArgs a;
     uniform_copy(&a, L.inst + 2 * wi.inst + par);
     a.n0 = n0;
-    zstream_tile<B, R, W, kMat, S>(a, wi.tile);
+    int tx, ty, tz;
+    tile_coords(a, wi.tile, tx, ty, tz);
+    zstream_tile<B, R, W, kMat, S, true>(a, tx, ty, tz);
   }
 }
 
@@ -585,41 +628,51 @@ hipError_t launch_batch_rw(int blt, MatKind mk, const BatchLaunch& L, int nblock
 }
 
 template <int B, int R, int W, int kMat, class S>
-void launch_k(const ZArgs& a, hipStream_t stream) {
-  const unsigned blocks = (unsigned)((long long)a.ntx * a.nty * a.ntz);
-  hipLaunchKernelGGL((zstream_kernel<B, R, W, kMat, S>), dim3(blocks), dim3(64 * W), 0, stream,
-                     a);
+void launch_k(const ZArgs& a, const ZPart& part, hipStream_t stream) {
+  const unsigned n = part.list ? (unsigned)part.nlist
+                               : (unsigned)((long long)a.ntx * a.nty * a.ntz);
+  if (part.form == kZLean) {
+    // the lean form is built for the planner's 4 x 8 tile only
+    if constexpr (R == 4 && W == 8)
+      hipLaunchKernelGGL((zstream_lean_kernel<B, R, W, kMat, S>), dim3(n), dim3(64 * W), 0,
+                         stream, a, part.list);
+  } else {
+    hipLaunchKernelGGL((zstream_kernel<B, R, W, kMat, S>), dim3(n), dim3(64 * W), 0, stream, a,
+                       part.list);
+  }
 }
 
 template <int B, int R, int W>
-hipError_t launch_one(const ZArgs& a, Storage st, MatKind mk, hipStream_t stream) {
+hipError_t launch_one(const ZArgs& a, Storage st, MatKind mk, const ZPart& part,
+                      hipStream_t stream) {
   if (st == kF16) {
     if (mk == kMatIdx)
-      launch_k<B, R, W, kMatIdx, _Float16>(a, stream);
+      launch_k<B, R, W, kMatIdx, _Float16>(a, part, stream);
     else
-      launch_k<B, R, W, kMatIso, _Float16>(a, stream);
+      launch_k<B, R, W, kMatIso, _Float16>(a, part, stream);
   } else if (st == kBF16) {
     if (mk == kMatIdx)
-      launch_k<B, R, W, kMatIdx, __bf16>(a, stream);
+      launch_k<B, R, W, kMatIdx, __bf16>(a, part, stream);
     else
-      launch_k<B, R, W, kMatIso, __bf16>(a, stream);
+      launch_k<B, R, W, kMatIso, __bf16>(a, part, stream);
   } else if (mk == kMatFull) {
-    launch_k<B, R, W, kMatFull, float>(a, stream);
+    launch_k<B, R, W, kMatFull, float>(a, part, stream);
   } else if (mk == kMatIdx) {
-    launch_k<B, R, W, kMatIdx, float>(a, stream);
+    launch_k<B, R, W, kMatIdx, float>(a, part, stream);
   } else {
-    launch_k<B, R, W, kMatIso, float>(a, stream);
+    launch_k<B, R, W, kMatIso, float>(a, part, stream);
   }
   return hipGetLastError();
 }
 
 template <int R, int W>
-hipError_t launch_rw(int blt, const ZArgs& a, Storage st, MatKind mk, hipStream_t stream) {
+hipError_t launch_rw(int blt, const ZArgs& a, Storage st, MatKind mk, const ZPart& part,
+                     hipStream_t stream) {
   switch (blt) {
-    case 1: return launch_one<1, R, W>(a, st, mk, stream);
-    case 2: return launch_one<2, R, W>(a, st, mk, stream);
-    case 3: return launch_one<3, R, W>(a, st, mk, stream);
-    case 4: return launch_one<4, R, W>(a, st, mk, stream);
+    case 1: return launch_one<1, R, W>(a, st, mk, part, stream);
+    case 2: return launch_one<2, R, W>(a, st, mk, part, stream);
+    case 3: return launch_one<3, R, W>(a, st, mk, part, stream);
+    case 4: return launch_one<4, R, W>(a, st, mk, part, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -689,11 +742,55 @@ hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* f
   return hipGetLastError();
 }
 
+bool zstream_lean_supported(const ZTiling& t) { return t.rows == 4 && t.waves == 8; }
+
+// Column width of the tile order (ZArgs::scw) for a storage format.
+int zstream_scw(Storage st) { return st == kF32 ? 10 : 0; }
+
+void zstream_classify(const ZTiling& t, Storage st, const Grid& g, const SourceDesc* src,
+                      int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean) {
+  ZArgs a{};
+  ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
+  a.scw = zstream_scw(st);
+  const int B = t.blt;
+  const int ox = 64 - 2 * B, oy = t.rows * t.waves - 2 * B;
+  extra.clear();
+  lean.clear();
+  const int ntiles = a.ntx * a.nty * a.ntz;
+  for (int pos = 0; pos < ntiles; ++pos) {
+    // the kernels' tile order, so that both lists keep neighbouring tiles
+    // together
+    int tx, ty, tz;
+    tile_coords(a, pos, tx, ty, tz);
+    // the tile's input window: x [x0, x0+64), y [y0, y0+rows*waves); its
+    // stream computes planes z_lo-2B .. z_hi-1+B
+    const int x0 = tx * ox - B, y0 = ty * oy - B;
+    const int z_lo = tz * t.zchunk, z_hi = std::min(g.nz, z_lo + t.zchunk);
+    bool x = false;
+    for (int q = 0; q < nsrc && !x; ++q)
+      x = src[q].i >= x0 && src[q].i < x0 + 64 && src[q].j >= y0 &&
+          src[q].j < y0 + t.rows * t.waves && src[q].k >= z_lo - 2 * B &&
+          src[q].k <= z_hi - 1 + B;
+    // owned cells on the monitored plane
+    if (!x && d.ncomp > 0) {
+      if (d.axis == 2)
+        x = d.index >= z_lo && d.index < z_hi;
+      else if (d.axis == 0)
+        x = d.index >= tx * ox && d.index < tx * ox + ox;
+      else
+        x = d.index >= ty * oy && d.index < ty * oy + oy;
+    }
+    (x ? extra : lean).push_back(tx + a.ntx * (ty + a.nty * tz));
+  }
+}
+
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,
                           const Sources& s, const Dft& d, int n0, int run_step0,
-                          hipStream_t stream) {
+                          const ZPart& part, hipStream_t stream) {
   if (!ztiling_supported(t, st, mk)) return hipErrorInvalidValue;
+  if (part.form == kZLean && !zstream_lean_supported(t)) return hipErrorInvalidValue;
+  if (part.list && part.nlist < 1) return hipErrorInvalidValue;
   // in-plane byte offsets are 32-bit
   if ((unsigned long long)g.sxy * 4ull >= (1ull << 32)) return hipErrorInvalidValue;
   ZArgs a;
@@ -706,20 +803,16 @@ hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const
   a.d = d;
   a.n0 = n0;
   a.run_step0 = run_step0;
-  const int ox = 64 - 2 * t.blt;
-  const int oy = t.rows * t.waves - 2 * t.blt;
-  a.ntx = (g.nx + ox - 1) / ox;
-  a.nty = (g.ny + oy - 1) / oy;
+  ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
   a.zc = t.zchunk;
-  a.ntz = (g.nz + t.zchunk - 1) / t.zchunk;
   // FP32 on rows of more than 10 tiles: columns 10 tiles wide keep the tiles
   // an XCD runs together in a compact block that shares halo rows in L2
   // (1024^3: ~200 instead of ~190 Gcell/s); 16-bit launches measured no gain
-  a.scw = st == kF32 ? 10 : 0;
-  if (t.rows == 2 && t.waves == 4) return launch_rw<2, 4>(t.blt, a, st, mk, stream);
-  if (t.rows == 2 && t.waves == 8) return launch_rw<2, 8>(t.blt, a, st, mk, stream);
-  if (t.rows == 4 && t.waves == 4) return launch_rw<4, 4>(t.blt, a, st, mk, stream);
-  return launch_rw<4, 8>(t.blt, a, st, mk, stream);
+  a.scw = zstream_scw(st);
+  if (t.rows == 2 && t.waves == 4) return launch_rw<2, 4>(t.blt, a, st, mk, part, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_rw<2, 8>(t.blt, a, st, mk, part, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_rw<4, 4>(t.blt, a, st, mk, part, stream);
+  return launch_rw<4, 8>(t.blt, a, st, mk, part, stream);
 }
 
 }  // namespace gd
