@@ -148,6 +148,33 @@ def test_wide_grids_column_tile_order(gpu, oracle, shape):
                 f"wide {shape} {t} DFT comp {c}"
 
 
+@pytest.mark.parametrize("case", ["no-extras", "all-extras", "split", "split-remainder"])
+def test_split_launch_forms(gpu, oracle, case):
+    """The planner's 4 x 8 tile runs a launch as the lean kernel alone (no
+    sources, no DFT), the full kernel alone (every tile holds the source or
+    the monitored plane), or both on two streams (the usual case, also with a
+    shorter last launch); every form == the oracle."""
+    shape = {"no-extras": (150, 70, 40), "all-extras": (40, 20, 12),
+             "split": (150, 70, 40), "split-remainder": (131, 61, 45)}[case]
+    T = {"split-remainder": 10}.get(case, 8)
+    eps, cb, db = iso_problem(gpu, shape, seed=5, nspheres=6, rmin=2, rmax=6)
+    setup = dict(cb=cb, db=db, fields=random_fields(shape, 17, scale=0.05))
+    if case != "no-extras":
+        src = gpu.gaussian_pulse_table(T, 3.0, 2.0, 0.07)
+        setup["sources"] = [(gpu.EZ, *centre(shape), src), (gpu.HX, 3, shape[1] - 4, 5, src)]
+        setup["dft"] = dict(comps=[1, 5], axis=1, index=shape[1] // 3,
+                            f_cps=np.array([0.04, 0.1]))
+    want, res = oracle_run(oracle, shape, setup, T)
+    say(f"[split] {case} {shape}")
+    got, dft = run_variant(gpu, shape, setup, T, gpu.TILED, None)
+    assert_fields_equal(got, want, f"split {case}")
+    if res is not None:
+        wre, wim = res
+        for n, c in enumerate(setup["dft"]["comps"]):
+            assert bits_equal(dft[c][0], wre[n]) and bits_equal(dft[c][1], wim[n]), \
+                f"split {case} DFT comp {c}"
+
+
 def test_random_initial_fields_full_material(gpu, oracle):
     """Random fields (boundary included) + random full coefficient arrays."""
     shape = (41, 38, 27)
