@@ -204,7 +204,10 @@ int gd_dft_disable(gd_ctx* ctx);
 /* ---- stepping ---------------------------------------------------------------- */
 
 /* Advance nsteps steps from the current step with the given kernel variant
- * (tiling may be NULL). Synchronous: returns after the device finished. */
+ * (tiling may be NULL). Synchronous: returns after the device finished. With
+ * the planner's 4 x 8 tile a TILED launch runs the tiles that hold a source
+ * cell or monitored cells on a second stream of the context, alongside the
+ * others; both streams have the default priority. */
 int gd_run(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
 /* The tiling the planner would use for GD_VARIANT_TILED on this context. */
 int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
