@@ -307,11 +307,9 @@ gd_tiling plan_blt(size_t nx, size_t ny, size_t nz, int storage, int blt) {
   size_t zc = nz * txy / 1024;
   if (zc < 16) zc = 16;
   if (zc > 64) zc = 64;
-  // binary16 launches are held by their arithmetic more than by HBM: a 128-plane
-  // chunk halves the recomputed z halo, worth it while the grid still gives
-  // at least 16 workgroups per CU (1024^3: 258 vs 251 Gcell/s; the FP32 form
-  // measured slower with longer chunks)
-  if (is16(storage) && txy * ((nz + 127) / 128) >= 4096) zc = 128;
+  // (64 also for 16-bit fields at 1024^3: 297 against 278 Gcell/s for
+  // 128-plane chunks since the lean kernel; with the source and DFT code in
+  // every tile the longer chunks had been faster)
   if (zc > nz) zc = nz;
   t.zchunk = (int)zc;
   return t;
