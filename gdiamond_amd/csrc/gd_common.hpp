@@ -320,6 +320,9 @@ struct NnFeat {
   float dx_over_dt;  // 1/eps = cb * dx/dt
 };
 hipError_t launch_restrict(const Fields& f, int nx, int ny, int nz, float* out, hipStream_t s);
+// out[q] = table[id[q]] for q < n: the dense Cb of an indexed material.
+hipError_t launch_expand_material(const uint8_t* id, const float* table, size_t n, float* out,
+                                  hipStream_t s);
 int nn_dims(const ::gd_nn* o, int* nx, int* ny, int* nz);
 // features -> network -> corrected fields, all on stream s
 hipError_t nn_correct(::gd_nn* o, const Fields& f, const float* pe0, const float* pe1,

@@ -336,6 +336,16 @@ __global__ __launch_bounds__(256) void nn_apply_kernel(Fields f, const float* __
   }
 }
 
+// Dense Cb of an indexed material: out[q] = table[id[q]] (the corrector's
+// features need Cb per cell).
+__global__ __launch_bounds__(256) void expand_material_kernel(const uint8_t* __restrict__ id,
+                                                              const float* __restrict__ table,
+                                                              long long n,
+                                                              float* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) out[q] = table[id[q]];
+}
+
 // Yee-consistent restriction of fine fields (cell size h) to a grid of
 // cell size 2h with cnx = nx/2, ... cells: E components average the two fine
 // edges along their axis, H components the four fine faces of their face.
@@ -724,6 +734,14 @@ hipError_t launch_restrict(const Fields& f, int nx, int ny, int nz, float* out, 
   if (cn == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((cn + 255) / 256)), dim3(256), 0, s, f, nx,
                      ny, nz, cnx, cny, cnz, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_material(const uint8_t* id, const float* table, size_t n, float* out,
+                                  hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(expand_material_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     id, table, (long long)n, out);
   return hipGetLastError();
 }
 

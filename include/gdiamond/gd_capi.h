@@ -122,8 +122,8 @@ int gd_get_storage(const gd_ctx* ctx, int* storage, int* scale_exp);
 /* Device formats of the isotropic material (gd_set_material_format), and what
  * gd_get_material_format reports. */
 enum {
-  GD_MATERIAL_AUTO = 0,    /* DENSE for gd_set_isotropic(_eps), INDEXED for
-                              gd_set_isotropic_indexed */
+  GD_MATERIAL_AUTO = 0,    /* INDEXED when Cb has at most GD_MAX_MATERIALS distinct
+                              values and the storage is not FP64, else DENSE */
   GD_MATERIAL_DENSE = 1,   /* one Cb per cell in FP32 (FP64 with GD_STORAGE_F64) */
   GD_MATERIAL_INDEXED = 2, /* one 1-byte material id per cell + an FP32 table of Cb values */
   GD_MATERIAL_FULL = 3     /* reported only: the 12-array form of gd_set_coefficients */
@@ -132,11 +132,11 @@ enum {
 
 /* Choose how the isotropic Cb is held on the device; must precede the
  * material (default GD_MATERIAL_AUTO). Both forms keep Cb exact and give
- * bit-identical results: DENSE reads 4 B per cell-update and is the faster
- * tiled kernel for every field format; INDEXED reads 1 B and holds a quarter
- * of the device memory. INDEXED is not available with GD_STORAGE_F64. No
- * counterpart in the reference, which streams 12 FP32 coefficient arrays
- * (gdiamond_gpu.cuh:871-887). */
+ * bit-identical results: DENSE reads 4 B of material per cell, INDEXED 1 B
+ * (and a table lookup per level), which runs faster (512^3 FP32 TILED: 220
+ * against 211 Gcell/s) and holds a quarter of the memory. INDEXED is not
+ * available with GD_STORAGE_F64. No counterpart in the reference, which
+ * streams 12 FP32 coefficient arrays (gdiamond_gpu.cuh:871-887). */
 int gd_set_material_format(gd_ctx* ctx, int format);
 /* Format in use (DENSE, INDEXED or FULL; -1 before any material) and the
  * number of table entries of the INDEXED form (0 otherwise). */
@@ -335,8 +335,8 @@ int gd_restrict_fields(gd_ctx* ctx, float* const out[6]);
  * forward, F += y / r on interior cells. cfg = {se, sh, sd, re, rh, dx/dt}:
  * feature scales of E, H, E - E_prev, output scales of E and H, and dx/dt of
  * the problem (1/eps = cb*dx/dt). Steps past the last full interval run
- * uncorrected. FP32 fields, dense isotropic material, network on the same
- * grid. */
+ * uncorrected. FP32 fields, isotropic material (dense or indexed), network
+ * on the same grid. */
 int gd_run_corrected(gd_ctx* ctx, gd_nn* nn, size_t nsteps, int interval, const float cfg[6],
                      const gd_tiling* tiling);
 /* D = A B for 16x16 matrices (row-major floats, rounded to BF16) computed

@@ -49,7 +49,7 @@ def _c3_context(gpu, cb, db, init, T):
     ctx = gpu.Context(N3, N3, N3)
     ctx.set_storage(gpu.F16, SCALE)
     ctx.set_isotropic(cb, db)
-    assert ctx.get_material_format()[0] == gpu.MAT_DENSE
+    assert ctx.get_material_format()[0] == gpu.MAT_INDEXED
     c = N3 // 2
     ctx.add_source(gpu.EZ, c, c, c, gpu.gaussian_pulse_table(T, 30.0, 10.0, 0.05))
     ctx.set_fields(init)

@@ -148,20 +148,20 @@ def test_material_formats(gd, oracle):
     cb, db, src, dft = _f16_problem(gd, shape, T, seed=12)
     distinct = np.unique(cb.view(np.uint32)).size
     assert 2 <= distinct <= 256
-    # AUTO picks the dense form for every field format; INDEXED holds one table
-    # entry per distinct value
+    # AUTO picks the indexed form for a sphere medium, with one table entry per
+    # distinct value, for FP32 and 16-bit fields; DENSE on request
     with gd.Context(*shape) as ctx:
         ctx.set_storage(gd.F16, [0] * 6)
         ctx.set_isotropic(cb, db)
-        assert ctx.get_material_format() == (gd.MAT_DENSE, 0)
-    with gd.Context(*shape) as ctx:
-        ctx.set_storage(gd.BF16, [0] * 6)
-        ctx.set_material_format(gd.MAT_INDEXED)
-        ctx.set_isotropic(cb, db)
         assert ctx.get_material_format() == (gd.MAT_INDEXED, distinct)
     with gd.Context(*shape) as ctx:
+        ctx.set_storage(gd.BF16, [0] * 6)
+        ctx.set_material_format(gd.MAT_DENSE)
         ctx.set_isotropic(cb, db)
         assert ctx.get_material_format() == (gd.MAT_DENSE, 0)
+    with gd.Context(*shape) as ctx:
+        ctx.set_isotropic(cb, db)
+        assert ctx.get_material_format() == (gd.MAT_INDEXED, distinct)
         with pytest.raises(gd.GdError):
             ctx.set_material_format(gd.MAT_DENSE)  # after the material
     # a continuous medium has too many values: AUTO -> dense, INDEXED -> error
