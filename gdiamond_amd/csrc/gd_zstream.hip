@@ -35,6 +35,12 @@
 // DFT snapshots hold the FP32 value of every level. Every workgroup that
 // computes a cell at some level does so from the same decoded inputs with the
 // same operations, so overlapping tiles agree bit for bit.
+//
+// Two builds of the tile: zstream_kernel with the point-source and DFT code,
+// and zstream_lean_kernel without it. The host sorts the tiles of a launch
+// (zstream_classify) and runs the few that need that code with the full
+// kernel next to the lean kernel over the rest; the result is the same as one
+// full launch, since the code it leaves out does nothing in those tiles.
 #include <algorithm>
 #include <type_traits>
 
