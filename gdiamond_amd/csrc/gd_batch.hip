@@ -25,6 +25,9 @@ struct Inst {
   int cls = 0;
   float* f[2][6] = {{nullptr}};  // field copies (in the arena)
   float* cb = nullptr;           // dense Cb (in the arena)
+  uint8_t* id = nullptr;         // material id per cell (in the arena)
+  float* table = nullptr;        // Cb per id, kMaxMaterials entries (in the arena)
+  bool indexed = false;          // id/table hold the material (<= kMaxMaterials values)
   float db = 0.0f;
   bool have_mat = false;
   struct Src {
@@ -86,6 +89,13 @@ gd_tiling default_tiling(size_t nx, size_t ny, size_t nz) {
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Arena bytes of one instance: 12 field arrays and the dense Cb (FP32), the
+// material ids (1 byte per cell) and the id table.
+size_t inst_bytes(size_t N) {
+  return 13 * align_up(N * sizeof(float), 256) + align_up(N, 256) +
+         align_up(gd::kMaxMaterials * sizeof(float), 256);
+}
 
 }  // namespace
 
@@ -162,6 +172,14 @@ void invalidate(gd_batch* b) {
   b->plan_valid = false;
 }
 
+// Material form of the batch's launches: the indexed one when every instance
+// has it, else the dense one.
+gd::MatKind batch_mat(const gd_batch* b) {
+  for (const auto& in : b->inst)
+    if (!in.indexed) return gd::kMatIso;
+  return gd::kMatIdx;
+}
+
 gd::ZArgs make_args(const gd_batch* b, int i, int parity, const gd::ZTiling& t) {
   const Inst& in = b->inst[i];
   gd::ZArgs a{};
@@ -174,6 +192,8 @@ gd::ZArgs make_args(const gd_batch* b, int i, int parity, const gd::ZTiling& t) 
     a.sc.inv[c] = 1.0f;
   }
   a.m.cb = in.cb;
+  a.m.id = in.id;
+  a.m.table = in.table;
   a.m.db = in.db;
   a.m.db64 = (double)in.db;
   a.g.nx = (int)in.nx;
@@ -218,7 +238,7 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
     BH(b, hipMemcpy(p.d_items, items.data(), items.size() * sizeof(gd::BatchItem),
                     hipMemcpyHostToDevice));
   p.nitems = (int)items.size();
-  p.nblocks = gd::zstream_batch_blocks(t, gd::kMatIso, b->device);
+  p.nblocks = gd::zstream_batch_blocks(t, batch_mat(b), b->device);
   if (p.nblocks < 1) return b->fail(GD_ERR_HIP, "occupancy query failed");
   return GD_OK;
 }
@@ -369,7 +389,7 @@ int issue_run(gd_batch* b, size_t nsteps, Schedule& s, size_t* nlaunch) {
       Lh.parity = b->d_parity + gi;
       Lh.step_offset = k * g.blt;
       Lh.launch_index = k;
-      BH(b, gd::launch_zstream_batch(p->t, gd::kMatIso, Lh, p->nblocks, gs));
+      BH(b, gd::launch_zstream_batch(p->t, batch_mat(b), Lh, p->nblocks, gs));
       ++nl;
     }
     coff += L;
@@ -421,7 +441,7 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
     in.ny = dims[3 * i + 1];
     in.nz = dims[3 * i + 2];
     in.N = in.nx * in.ny * in.nz;
-    bytes += 13 * align_up(in.N * sizeof(float), 256);
+    bytes += inst_bytes(in.N);
     if (bytes > (size_t(1) << 50)) {  // far above any device; also keeps the sum from wrapping
       g_batch_create_error = "batch larger than 1 PiB";
       delete b;
@@ -461,6 +481,10 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
       for (int c = 0; c < 6; ++c, p += a) in.f[k][c] = reinterpret_cast<float*>(p);
     in.cb = reinterpret_cast<float*>(p);
     p += a;
+    in.id = reinterpret_cast<uint8_t*>(p);
+    p += align_up(in.N, 256);
+    in.table = reinterpret_cast<float*>(p);
+    p += align_up(gd::kMaxMaterials * sizeof(float), 256);
   }
   if (hipMalloc(&b->d_srcdesc, (size_t)ninst * gd::kMaxSources * sizeof(gd::SourceDesc)) !=
           hipSuccess ||
@@ -507,7 +531,21 @@ int gd_batch_set_isotropic(gd_batch* b, int i, const float* cb, float db) {
   BH(b, hipStreamSynchronize(b->stream));
   Inst& in = b->inst[i];
   BH(b, hipMemcpy(in.cb, cb, in.N * sizeof(float), hipMemcpyHostToDevice));
-  if (in.db != db) invalidate(b);  // db lives in the launch arguments
+  // the indexed form as well when the medium has few enough values: a
+  // launch runs it when every instance has it (1 byte of material per cell
+  // instead of 4; bit-identical results)
+  std::vector<uint8_t> id;
+  std::vector<float> table;
+  const bool indexed = gd::index_materials(cb, in.N, id, table);
+  if (indexed) {
+    table.resize(gd::kMaxMaterials, 0.0f);
+    BH(b, hipMemcpy(in.id, id.data(), in.N, hipMemcpyHostToDevice));
+    BH(b, hipMemcpy(in.table, table.data(), gd::kMaxMaterials * sizeof(float),
+                    hipMemcpyHostToDevice));
+  }
+  // db lives in the launch arguments, the material form in the kernel choice
+  if (in.db != db || in.indexed != indexed) invalidate(b);
+  in.indexed = indexed;
   in.db = db;
   in.have_mat = true;
   return GD_OK;
@@ -693,7 +731,8 @@ int gd_batch_autotune(gd_batch* b, int reps) {
             L.parity = b->d_parity;  // 0: read copy 0, write copy 1
             L.step_offset = 0;
             L.launch_index = 0;
-            if (e == hipSuccess) e = gd::launch_zstream_batch(t, gd::kMatIso, L, p.nblocks, b->stream);
+            if (e == hipSuccess)
+              e = gd::launch_zstream_batch(t, batch_mat(b), L, p.nblocks, b->stream);
           }
           if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
           if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
@@ -752,7 +791,7 @@ int gd_batch_run(gd_batch* b, size_t nsteps, int mode) {
           const int blt = (int)std::min<size_t>((size_t)g.blt, nsteps - (size_t)k * g.blt);
           gd::ZTiling t{blt, g.rows, g.waves, b->cls[in.cls].t.zchunk};
           gd::ZArgs a = make_args(b, i, par, t);
-          BH(b, gd::launch_zstream(t, gd::kF32, a.src, a.dst, a.sc, a.m, gd::kMatIso, a.g, a.s,
+          BH(b, gd::launch_zstream(t, gd::kF32, a.src, a.dst, a.sc, a.m, batch_mat(b), a.g, a.s,
                                    a.d, (int)(b->step + (size_t)k * g.blt), 0,
                                    gd::ZPart{gd::kZFull, nullptr, 0}, b->stream));
           par ^= 1;

@@ -227,39 +227,6 @@ void free_material(gd_ctx* c) {
   c->have_mat = false;
 }
 
-// Material ids of an isotropic Cb array when it holds at most
-// GD_MAX_MATERIALS distinct bit patterns (ids in order of first appearance).
-// Returns false when there are more.
-bool index_materials(const float* cb, size_t n, std::vector<uint8_t>& id,
-                     std::vector<float>& table) {
-  constexpr unsigned kSlots = 1024;  // open addressing, load <= 1/4
-  uint32_t key[kSlots];
-  int val[kSlots];
-  for (unsigned q = 0; q < kSlots; ++q) val[q] = -1;
-  id.resize(n);
-  table.clear();
-  uint32_t last_bits = 0;
-  int last_id = -1;
-  for (size_t q = 0; q < n; ++q) {
-    uint32_t bits;
-    std::memcpy(&bits, &cb[q], 4);
-    if (last_id < 0 || bits != last_bits) {
-      unsigned h = (bits * 2654435761u) >> 22;
-      while (val[h] >= 0 && key[h] != bits) h = (h + 1) & (kSlots - 1);
-      if (val[h] < 0) {
-        if ((int)table.size() == GD_MAX_MATERIALS) return false;
-        key[h] = bits;
-        val[h] = (int)table.size();
-        table.push_back(cb[q]);
-      }
-      last_bits = bits;
-      last_id = val[h];
-    }
-    id[q] = (uint8_t)last_id;
-  }
-  return true;
-}
-
 int upload_indexed(gd_ctx* c, const uint8_t* id, const float* table, int ntable, float db) {
   free_material(c);
   hipError_t e = hipMalloc(&c->mat_id, c->N);
@@ -526,7 +493,7 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
       return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
     std::vector<uint8_t> id;
     std::vector<float> table;
-    if (index_materials(cb, c->N, id, table))
+    if (gd::index_materials(cb, c->N, id, table))
       return upload_indexed(c, id.data(), table.data(), (int)table.size(), db);
     if (c->mat_req == GD_MATERIAL_INDEXED)
       return c->fail(GD_ERR_UNSUPPORTED, "more than 256 distinct Cb values for the indexed "

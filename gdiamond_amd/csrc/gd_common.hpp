@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <vector>
 
 namespace gd {
@@ -53,6 +54,39 @@ struct Scales {
 //             table of at most 256 Cb values (exact Cb, 1 byte per cell).
 enum MatKind { kMatIso = 0, kMatFull = 1, kMatIdx = 2 };
 constexpr int kMaxMaterials = 256;
+
+// Material ids of an isotropic Cb array when it holds at most
+// kMaxMaterials distinct bit patterns (ids in order of first appearance).
+// Returns false when there are more.
+inline bool index_materials(const float* cb, size_t n, std::vector<uint8_t>& id,
+                            std::vector<float>& table) {
+  constexpr unsigned kSlots = 1024;  // open addressing, load <= 1/4
+  uint32_t key[kSlots];
+  int val[kSlots];
+  for (unsigned q = 0; q < kSlots; ++q) val[q] = -1;
+  id.resize(n);
+  table.clear();
+  uint32_t last_bits = 0;
+  int last_id = -1;
+  for (size_t q = 0; q < n; ++q) {
+    uint32_t bits;
+    std::memcpy(&bits, &cb[q], 4);
+    if (last_id < 0 || bits != last_bits) {
+      unsigned h = (bits * 2654435761u) >> 22;
+      while (val[h] >= 0 && key[h] != bits) h = (h + 1) & (kSlots - 1);
+      if (val[h] < 0) {
+        if ((int)table.size() == kMaxMaterials) return false;
+        key[h] = bits;
+        val[h] = (int)table.size();
+        table.push_back(cb[q]);
+      }
+      last_bits = bits;
+      last_id = val[h];
+    }
+    id[q] = (uint8_t)last_id;
+  }
+  return true;
+}
 
 struct Material {
   const void* c[12];

@@ -124,6 +124,29 @@ def test_batch_autotune_keeps_state(gd, oracle):
         b.close()
 
 
+@pytest.mark.parametrize("continuous", [False, True])
+def test_batch_material_forms(gd, oracle, continuous):
+    """Launches run the 1-byte indexed material when every instance's Cb has
+    at most 256 values (sphere media), the dense FP32 Cb as soon as one
+    instance has more (a continuous medium); both == the oracle."""
+    dims = DIMS[:3]
+    T = 11
+    b, probs = _make_batch(gd, dims, T, init=True)
+    try:
+        if continuous:
+            d = dims[1]
+            rng = np.random.default_rng(7)
+            cb = (DT_C1 / rng.uniform(1.0, 6.0, (d[2], d[1], d[0]))).astype(np.float32)
+            assert np.unique(cb).size > 256
+            b.set_isotropic(1, cb, probs[1][1])
+            probs[1] = (cb,) + probs[1][1:]
+        say(f"[batch material] continuous={continuous}")
+        b.run(T, gd.BATCH_GRAPH)
+        _check(b, _oracle(oracle, dims, probs, T), f"material continuous={continuous}")
+    finally:
+        b.close()
+
+
 def test_batch_errors(gd):
     b = gd.Batch([(20, 20, 20), (24, 20, 16)])
     try:
