@@ -48,15 +48,22 @@ struct SizeClass {
 };
 
 // One launch configuration (a tiling) over a set of instances: per-instance
-// launch arguments for both copy parities and the work items.
+// launch arguments for both copy parities and the work items. Items whose
+// tile holds a source cell come first (nextra of them, run by the full batch
+// kernel); the others run the lean build without source code.
 struct Plan {
   gd::ZTiling t{};
   gd::ZArgs* d_args = nullptr;   // [2 * ninst]
   gd::BatchItem* d_items = nullptr;
   int nitems = 0;
-  int nblocks = 0;
+  int nextra = 0;
+  int nblocks = 0;       // full kernel
+  int nblocks_lean = 0;  // lean kernel
   std::vector<int> insts;
 };
+
+// Queue counters of one launch: one for the full kernel, one for the lean one.
+constexpr int kCountersPerLaunch = 2;
 
 struct Group {
   int blt = 0, rows = 0, waves = 0;
@@ -220,7 +227,11 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
   const size_t ninst = b->inst.size();
   std::vector<gd::ZArgs> args(2 * ninst);
   std::memset(args.data(), 0, args.size() * sizeof(gd::ZArgs));
-  std::vector<gd::BatchItem> items;
+  std::vector<gd::BatchItem> items, lean_items;
+  std::vector<int> extra_ids, lean_ids;
+  std::vector<gd::SourceDesc> sd;
+  gd::Dft no_dft{};
+  no_dft.ncomp = 0;
   for (int i : insts) {
     gd::ZTiling ti = t;
     if (per_class) ti.zchunk = b->cls[b->inst[i].cls].t.zchunk;
@@ -228,8 +239,17 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
     const gd::ZArgs& a = args[2 * i];
     const long long nt = (long long)a.ntx * a.nty * a.ntz;
     if (nt > (1ll << 30)) return b->fail(GD_ERR_ARG, "too many tiles");
-    for (int q = 0; q < (int)nt; ++q) items.push_back(gd::BatchItem{i, q});
+    // tiles with a source cell in their window need the full kernel (the
+    // batch tile order is x fastest, then y, then z: ids are positions)
+    sd.clear();
+    for (const auto& s : b->inst[i].src)
+      sd.push_back(gd::SourceDesc{s.comp, (int)s.i, (int)s.j, (int)s.k, s.table});
+    gd::zstream_classify(ti, 0, a.g, sd.data(), (int)sd.size(), no_dft, extra_ids, lean_ids);
+    for (int q : extra_ids) items.push_back(gd::BatchItem{i, q});
+    for (int q : lean_ids) lean_items.push_back(gd::BatchItem{i, q});
   }
+  p.nextra = (int)items.size();
+  items.insert(items.end(), lean_items.begin(), lean_items.end());
   if (items.size() > (size_t)INT32_MAX) return b->fail(GD_ERR_ARG, "too many work items");
   BH(b, hipMalloc(&p.d_args, args.size() * sizeof(gd::ZArgs)));
   BH(b, hipMalloc(&p.d_items, std::max<size_t>(1, items.size()) * sizeof(gd::BatchItem)));
@@ -238,8 +258,9 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
     BH(b, hipMemcpy(p.d_items, items.data(), items.size() * sizeof(gd::BatchItem),
                     hipMemcpyHostToDevice));
   p.nitems = (int)items.size();
-  p.nblocks = gd::zstream_batch_blocks(t, batch_mat(b), b->device);
-  if (p.nblocks < 1) return b->fail(GD_ERR_HIP, "occupancy query failed");
+  p.nblocks = gd::zstream_batch_blocks(t, batch_mat(b), false, b->device);
+  p.nblocks_lean = gd::zstream_batch_blocks(t, batch_mat(b), true, b->device);
+  if (p.nblocks < 1 || p.nblocks_lean < 1) return b->fail(GD_ERR_HIP, "occupancy query failed");
   return GD_OK;
 }
 
@@ -335,7 +356,7 @@ int get_schedule(gd_batch* b, size_t nsteps, Schedule** out) {
   for (auto& g : b->groups) {
     const int L = (int)((nsteps + g.blt - 1) / g.blt);
     s.nlaunch.push_back(L);
-    s.ncounters += L;
+    s.ncounters += kCountersPerLaunch * L;
     flip.push_back(L & 1);
     if (nsteps % g.blt) {
       int rc;
@@ -364,6 +385,31 @@ int ensure_streams(gd_batch* b, size_t n) {
   return GD_OK;
 }
 
+// One launch of plan p: the full kernel over the items with source tiles,
+// then the lean kernel over the others, on `stream`, each with its queue
+// counter (counters[0], counters[1]). L carries the other launch fields; nk
+// counts the kernels issued.
+int launch_plan(gd_batch* b, const Plan& p, gd::BatchLaunch L, unsigned* counters,
+                hipStream_t stream, size_t* nk) {
+  const gd::MatKind mk = batch_mat(b);
+  L.inst = p.d_args;
+  if (p.nextra > 0) {
+    L.items = p.d_items;
+    L.nitems = p.nextra;
+    L.counter = counters;
+    BH(b, gd::launch_zstream_batch(p.t, mk, false, L, p.nblocks, stream));
+    ++*nk;
+  }
+  if (p.nitems > p.nextra) {
+    L.items = p.d_items + p.nextra;
+    L.nitems = p.nitems - p.nextra;
+    L.counter = counters + 1;
+    BH(b, gd::launch_zstream_batch(p.t, mk, true, L, p.nblocks_lean, stream));
+    ++*nk;
+  }
+  return GD_OK;
+}
+
 // Issues the persistent launches of one run (every group on its own stream,
 // forked from and joined back into the main stream) and the advance kernel.
 int issue_run(gd_batch* b, size_t nsteps, Schedule& s, size_t* nlaunch) {
@@ -381,18 +427,16 @@ int issue_run(gd_batch* b, size_t nsteps, Schedule& s, size_t* nlaunch) {
       Plan* p = plan_for(b, g, blt, &rc);
       if (!p) return rc;
       gd::BatchLaunch Lh{};
-      Lh.inst = p->d_args;
-      Lh.items = p->d_items;
-      Lh.nitems = p->nitems;
-      Lh.counter = s.d_counters + coff + k;
       Lh.step = b->d_step;
       Lh.parity = b->d_parity + gi;
       Lh.step_offset = k * g.blt;
       Lh.launch_index = k;
-      BH(b, gd::launch_zstream_batch(p->t, batch_mat(b), Lh, p->nblocks, gs));
+      size_t nk = 0;
+      rc = launch_plan(b, *p, Lh, s.d_counters + coff + kCountersPerLaunch * k, gs, &nk);
+      if (rc != GD_OK) return rc;
       ++nl;
     }
-    coff += L;
+    coff += kCountersPerLaunch * L;
     BH(b, hipEventRecord(b->join_ev[gi], gs));
     BH(b, hipStreamWaitEvent(b->stream, b->join_ev[gi], 0));
   }
@@ -694,7 +738,8 @@ int gd_batch_autotune(gd_batch* b, int reps) {
   if (rc != GD_OK) return rc;
   invalidate(b);
   unsigned* d_cnt = nullptr;
-  BH(b, hipMalloc(&d_cnt, (size_t)(reps + 1) * sizeof(unsigned)));
+  const size_t ncnt = (size_t)kCountersPerLaunch * (reps + 1);
+  BH(b, hipMalloc(&d_cnt, ncnt * sizeof(unsigned)));
   const int rw[4][2] = {{2, 4}, {2, 8}, {4, 4}, {4, 8}};
   for (auto& sc : b->cls) {
     std::vector<int> insts = sc.insts;
@@ -719,20 +764,23 @@ int gd_batch_autotune(gd_batch* b, int reps) {
             (void)hipFree(d_cnt);
             return rc;
           }
-          hipError_t e = hipMemsetAsync(d_cnt, 0, (size_t)(reps + 1) * sizeof(unsigned), b->stream);
+          hipError_t e = hipMemsetAsync(d_cnt, 0, ncnt * sizeof(unsigned), b->stream);
+          size_t nl = 0;
           for (int r = 0; r <= reps && e == hipSuccess; ++r) {
             if (r == 1) e = hipEventRecord(b->ev0, b->stream);
             gd::BatchLaunch L{};
-            L.inst = p.d_args;
-            L.items = p.d_items;
-            L.nitems = p.nitems;
-            L.counter = d_cnt + r;
             L.step = b->d_step;
             L.parity = b->d_parity;  // 0: read copy 0, write copy 1
             L.step_offset = 0;
             L.launch_index = 0;
-            if (e == hipSuccess)
-              e = gd::launch_zstream_batch(t, batch_mat(b), L, p.nblocks, b->stream);
+            if (e == hipSuccess) {
+              rc = launch_plan(b, p, L, d_cnt + kCountersPerLaunch * r, b->stream, &nl);
+              if (rc != GD_OK) {
+                free_plan(p);
+                (void)hipFree(d_cnt);
+                return rc;
+              }
+            }
           }
           if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
           if (e == hipSuccess) e = hipEventSynchronize(b->ev1);

@@ -957,7 +957,8 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     // the context's stream (ahead of the launches that read them)
     std::vector<int> lean_ids;
     auto classify = [&](const gd::ZTiling& t, gd_ctx::Split& sp) -> int {
-      gd::zstream_classify(t, st, g, c->src_desc_host.data(), s.n, d, sp.ids, lean_ids);
+      gd::zstream_classify(t, gd::zstream_scw(st), g, c->src_desc_host.data(), s.n, d, sp.ids,
+                           lean_ids);
       sp.nextra = sp.ids.size();
       sp.ids.insert(sp.ids.end(), lean_ids.begin(), lean_ids.end());
       const size_t n = sp.ids.size();

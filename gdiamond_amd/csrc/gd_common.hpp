@@ -312,10 +312,11 @@ struct BatchLaunch {
   int step_offset;       // steps of the run before this launch
   int launch_index;      // launches of the group before this one in the run
 };
-hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
-                                hipStream_t stream);
+// lean: the items' tiles hold no source cell (the build without source code).
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, bool lean, const BatchLaunch& L,
+                                int nblocks, hipStream_t stream);
 // Workgroups of one batch launch that fill the device (occupancy x CUs).
-int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device);
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, bool lean, int device);
 // End of a run: step += nsteps, parity[g] ^= flip[g], every queue counter = 0.
 hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* flip, int ngroups,
                                 unsigned* counters, int ncounters, hipStream_t stream);
@@ -333,12 +334,15 @@ struct ZPart {
 };
 // The lean form exists for the 4 x 8 tile (the planner's).
 bool zstream_lean_supported(const ZTiling& t);
+// Column width of the tile order (ZArgs::scw) of single-problem launches for
+// a storage format; batch launches use 0 (x fastest along whole rows).
+int zstream_scw(Storage st);
 // Host: splits the tiles of tiling t into those that need the source or DFT
 // code (a source cell in the tile's input window and streamed z range, or
-// owned cells on the monitored plane) and the rest, each list in the
-// kernels' tile order.
-void zstream_classify(const ZTiling& t, Storage st, const Grid& g, const SourceDesc* src,
-                      int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean);
+// owned cells on the monitored plane) and the rest, each list of tile ids in
+// the tile order of column width scw.
+void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src, int nsrc,
+                      const Dft& d, std::vector<int>& extra, std::vector<int>& lean);
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,
                           const Sources& s, const Dft& d, int n0, int run_step0,

@@ -549,7 +549,8 @@ __device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
 // launch's queue until it runs dry (every wave of a workgroup sees the same
 // item, so all of them leave the loop together). Items of the largest
 // instances come first, so the tail of the launch is made of small tiles.
-template <int B, int R, int W, int kMat, class S>
+// kX = false: the items' tiles hold no source cell (the lean build).
+template <int B, int R, int W, int kMat, class S, bool kX>
 __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_batch_kernel(
     BatchLaunch L) {
   __shared__ int s_item;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_b
     a.n0 = n0;
     int tx, ty, tz;
     tile_coords(a, wi.tile, tx, ty, tz);
-    zstream_tile<B, R, W, kMat, S, true>(a, tx, ty, tz);
+    zstream_tile<B, R, W, kMat, S, kX>(a, tx, ty, tz);
   }
 }
 
@@ -584,53 +585,64 @@ __global__ __launch_bounds__(256) void batch_advance_kernel(int* step, int nstep
   for (int c = t; c < ncounters; c += blockDim.x) counters[c] = 0u;
 }
 
-template <int B, int R, int W, int kMat>
+template <int B, int R, int W, int kMat, bool kX>
 void launch_batch_k(const BatchLaunch& L, int nblocks, hipStream_t stream) {
-  hipLaunchKernelGGL((zstream_batch_kernel<B, R, W, kMat, float>), dim3(nblocks), dim3(64 * W), 0,
-                     stream, L);
+  hipLaunchKernelGGL((zstream_batch_kernel<B, R, W, kMat, float, kX>), dim3(nblocks),
+                     dim3(64 * W), 0, stream, L);
 }
 
-template <int B, int R, int W>
+template <int B, int R, int W, bool kX>
 const void* batch_kernel_fn(MatKind mk) {
-  return mk == kMatIdx ? reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIdx, float>)
-                       : reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIso, float>);
+  return mk == kMatIdx
+             ? reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIdx, float, kX>)
+             : reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIso, float, kX>);
 }
 
-template <int R, int W>
+template <int R, int W, bool kX>
 const void* batch_kernel_rw(int blt, MatKind mk) {
   switch (blt) {
-    case 1: return batch_kernel_fn<1, R, W>(mk);
-    case 2: return batch_kernel_fn<2, R, W>(mk);
-    case 3: return batch_kernel_fn<3, R, W>(mk);
-    case 4: return batch_kernel_fn<4, R, W>(mk);
+    case 1: return batch_kernel_fn<1, R, W, kX>(mk);
+    case 2: return batch_kernel_fn<2, R, W, kX>(mk);
+    case 3: return batch_kernel_fn<3, R, W, kX>(mk);
+    case 4: return batch_kernel_fn<4, R, W, kX>(mk);
     default: return nullptr;
   }
 }
 
+template <bool kX>
 const void* batch_kernel(const ZTiling& t, MatKind mk) {
-  if (t.rows == 2 && t.waves == 4) return batch_kernel_rw<2, 4>(t.blt, mk);
-  if (t.rows == 2 && t.waves == 8) return batch_kernel_rw<2, 8>(t.blt, mk);
-  if (t.rows == 4 && t.waves == 4) return batch_kernel_rw<4, 4>(t.blt, mk);
-  if (t.rows == 4 && t.waves == 8) return batch_kernel_rw<4, 8>(t.blt, mk);
+  if (t.rows == 2 && t.waves == 4) return batch_kernel_rw<2, 4, kX>(t.blt, mk);
+  if (t.rows == 2 && t.waves == 8) return batch_kernel_rw<2, 8, kX>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 4) return batch_kernel_rw<4, 4, kX>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 8) return batch_kernel_rw<4, 8, kX>(t.blt, mk);
   return nullptr;
 }
 
-template <int R, int W>
+template <int R, int W, bool kX>
 hipError_t launch_batch_rw(int blt, MatKind mk, const BatchLaunch& L, int nblocks,
                            hipStream_t stream) {
   const bool idx = mk == kMatIdx;
   switch (blt) {
-    case 1: idx ? launch_batch_k<1, R, W, kMatIdx>(L, nblocks, stream)
-                : launch_batch_k<1, R, W, kMatIso>(L, nblocks, stream); break;
-    case 2: idx ? launch_batch_k<2, R, W, kMatIdx>(L, nblocks, stream)
-                : launch_batch_k<2, R, W, kMatIso>(L, nblocks, stream); break;
-    case 3: idx ? launch_batch_k<3, R, W, kMatIdx>(L, nblocks, stream)
-                : launch_batch_k<3, R, W, kMatIso>(L, nblocks, stream); break;
-    case 4: idx ? launch_batch_k<4, R, W, kMatIdx>(L, nblocks, stream)
-                : launch_batch_k<4, R, W, kMatIso>(L, nblocks, stream); break;
+    case 1: idx ? launch_batch_k<1, R, W, kMatIdx, kX>(L, nblocks, stream)
+                : launch_batch_k<1, R, W, kMatIso, kX>(L, nblocks, stream); break;
+    case 2: idx ? launch_batch_k<2, R, W, kMatIdx, kX>(L, nblocks, stream)
+                : launch_batch_k<2, R, W, kMatIso, kX>(L, nblocks, stream); break;
+    case 3: idx ? launch_batch_k<3, R, W, kMatIdx, kX>(L, nblocks, stream)
+                : launch_batch_k<3, R, W, kMatIso, kX>(L, nblocks, stream); break;
+    case 4: idx ? launch_batch_k<4, R, W, kMatIdx, kX>(L, nblocks, stream)
+                : launch_batch_k<4, R, W, kMatIso, kX>(L, nblocks, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <bool kX>
+hipError_t launch_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
+                        hipStream_t stream) {
+  if (t.rows == 2 && t.waves == 4) return launch_batch_rw<2, 4, kX>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_batch_rw<2, 8, kX>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_batch_rw<4, 4, kX>(t.blt, mk, L, nblocks, stream);
+  return launch_batch_rw<4, 8, kX>(t.blt, mk, L, nblocks, stream);
 }
 
 template <int B, int R, int W, int kMat, class S>
@@ -720,18 +732,16 @@ bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk) {
   return t.rows * t.waves - 2 * t.blt >= 1 && 64 - 2 * t.blt >= 1;
 }
 
-hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
-                                hipStream_t stream) {
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, bool lean, const BatchLaunch& L,
+                                int nblocks, hipStream_t stream) {
   if (!ztiling_supported(t, kF32, mk) || mk == kMatFull || nblocks < 1)
     return hipErrorInvalidValue;
-  if (t.rows == 2 && t.waves == 4) return launch_batch_rw<2, 4>(t.blt, mk, L, nblocks, stream);
-  if (t.rows == 2 && t.waves == 8) return launch_batch_rw<2, 8>(t.blt, mk, L, nblocks, stream);
-  if (t.rows == 4 && t.waves == 4) return launch_batch_rw<4, 4>(t.blt, mk, L, nblocks, stream);
-  return launch_batch_rw<4, 8>(t.blt, mk, L, nblocks, stream);
+  return lean ? launch_batch<false>(t, mk, L, nblocks, stream)
+              : launch_batch<true>(t, mk, L, nblocks, stream);
 }
 
-int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device) {
-  const void* fn = batch_kernel(t, mk);
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, bool lean, int device) {
+  const void* fn = lean ? batch_kernel<false>(t, mk) : batch_kernel<true>(t, mk);
   if (!fn) return 0;
   int per_cu = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * t.waves, 0) != hipSuccess)
@@ -753,11 +763,11 @@ bool zstream_lean_supported(const ZTiling& t) { return t.rows == 4 && t.waves ==
 // Column width of the tile order (ZArgs::scw) for a storage format.
 int zstream_scw(Storage st) { return st == kF32 ? 10 : 0; }
 
-void zstream_classify(const ZTiling& t, Storage st, const Grid& g, const SourceDesc* src,
-                      int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean) {
+void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src, int nsrc,
+                      const Dft& d, std::vector<int>& extra, std::vector<int>& lean) {
   ZArgs a{};
   ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
-  a.scw = zstream_scw(st);
+  a.scw = scw;
   const int B = t.blt;
   const int ox = 64 - 2 * B, oy = t.rows * t.waves - 2 * B;
   extra.clear();
