@@ -265,7 +265,10 @@ enum {
 int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx);
 int gd_batch_destroy(gd_batch* b);
 const char* gd_batch_last_error(const gd_batch* b);
-/* Isotropic material of instance `inst` (as gd_set_isotropic / _eps). */
+/* Isotropic material of instance `inst` (as gd_set_isotropic / _eps). Launches
+ * use 1-byte material ids when every instance's Cb has at most
+ * GD_MAX_MATERIALS distinct values, the dense FP32 Cb otherwise (same
+ * results). */
 int gd_batch_set_isotropic(gd_batch* b, int inst, const float* cb, float db);
 int gd_batch_set_isotropic_eps(gd_batch* b, int inst, const float* eps_r, float dt);
 /* Point source of instance `inst` (as gd_add_source). */
@@ -287,7 +290,10 @@ int gd_batch_get_class_tiling(const gd_batch* b, int cls, gd_tiling* t);
  * and the step counter are left unchanged (the timing launches always read
  * the current copy). `reps` timed launches per candidate (>= 1). */
 int gd_batch_autotune(gd_batch* b, int reps);
-/* Advance every instance by nsteps (all source tables must cover them). */
+/* Advance every instance by nsteps (all source tables must cover them). Each
+ * persistent launch of B steps runs the items whose tile holds a source cell
+ * with the full kernel, then the other items with a build without source
+ * code, on the group's stream. */
 int gd_batch_run(gd_batch* b, size_t nsteps, int mode);
 /* Wall time of the last gd_batch_run on the device (events around the whole
  * run) and the number of kernel launches it made. */
