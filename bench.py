@@ -91,6 +91,23 @@ def cpu_baseline(gd, n, eps, dt, src, steps, threads):
     return n ** 3 * steps / dt_s / 1e9
 
 
+def reference_seq_rate(n=256, steps=4):
+    """The reference's own update_FDTD_seq (gdiamond.hpp:1164-1253), compiled
+    from its sources into oracle/_ref, on its default problem: the Mcells/s it
+    measures and prints itself, as Gcell-updates/s. None when that build is
+    absent."""
+    import ctypes as C
+    from oracle import oracle as O
+    try:
+        L = O.ref_lib()
+    except (OSError, FileNotFoundError):
+        return None
+    v = C.c_double(0.0)
+    if L.ref_seq_rate(n, n, n, steps, C.byref(v)) != 0:
+        return None
+    return v.value / 1e3
+
+
 def host_cpu():
     """CPU model name of the host, from /proc/cpuinfo (None where absent)."""
     try:
@@ -592,6 +609,14 @@ def main():
         out["cpu_baseline"]["single_thread"] = {
             "value": round(seq, 4), "unit": "Gcell-updates/s",
             "sample": f"steps 1..{seq_steps}, one thread (the update_FDTD_seq loop)"}
+        ref = reference_seq_rate()
+        if ref is not None:
+            out["cpu_baseline"]["reference_seq"] = {
+                "value": round(ref, 4), "unit": "Gcell-updates/s", "kind": "reference",
+                "cores": 1,
+                "sample": "the reference's own update_FDTD_seq built from its sources "
+                          "(oracle/_ref), its default problem at 256^3, 4 steps, the rate "
+                          "it prints itself"}
     print(json.dumps(out), flush=True)
 
 

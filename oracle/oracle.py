@@ -374,6 +374,7 @@ def ref_lib():
     L = C.CDLL(REF_LIB_PATH)
     L.ref_default_coefficients.argtypes = [C.c_int] * 3 + [C.POINTER(C.c_float)]
     L.ref_seq.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_float)]
+    L.ref_seq_rate.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_double)]
     L.ref_default_mz.argtypes = [C.c_int]
     L.ref_default_mz.restype = C.c_float
     L.ref_save_field_png.argtypes = [C.POINTER(C.c_float), C.c_char_p, C.c_int, C.c_int,

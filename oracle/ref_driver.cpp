@@ -48,6 +48,23 @@ int ref_seq(int nx, int ny, int nz, int T, float* out) {
   return 0;
 }
 
+// update_FDTD_seq(T) on the default problem, returning the rate the reference
+// itself measures and prints (gdiamond.hpp:1186, 1251-1253: Mcells/s over the
+// step loop); its console output is captured instead of printed.
+int ref_seq_rate(int nx, int ny, int nz, int T, double* mcells_per_s) {
+  gdiamond::gDiamond g(nx, ny, nz);
+  std::ostringstream out;
+  std::streambuf* old = std::cout.rdbuf(out.rdbuf());
+  g.update_FDTD_seq((size_t)T);
+  std::cout.rdbuf(old);
+  const std::string key = "seq performance (excluding figures output): ";
+  const std::string s = out.str();
+  const size_t at = s.find(key);
+  if (at == std::string::npos) return -1;
+  *mcells_per_s = std::strtod(s.c_str() + at + key.size(), nullptr);
+  return 0;
+}
+
 // The per-step source value update_FDTD_seq writes (gdiamond.hpp:1189-1191),
 // evaluated by the reference object's own members.
 float ref_default_mz(int t) {

@@ -103,3 +103,16 @@ def test_oracle_plants_detected(oracle):
     oracle.run(shape, f, coeffs=[g[n] for n in COEFF_NAMES], dx=np.float32(0.1),
                sources=[(4, i, j, k, g["Mz_table"])], nsteps=int(g["steps"]))
     assert not bits_equal(f[5], g["Hz"])
+
+
+def test_reference_build_reports_its_seq_rate(oracle):
+    """The reference's own update_FDTD_seq, built from its sources into
+    oracle/_ref, runs and reports the Mcells/s it prints (the bench's
+    reference CPU sample); skipped where that build is absent."""
+    import ctypes as C
+    import os
+    if not os.path.exists(oracle.REF_LIB_PATH):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    v = C.c_double(0.0)
+    assert oracle.ref_lib().ref_seq_rate(20, 18, 16, 3, C.byref(v)) == 0
+    assert v.value > 0.0
