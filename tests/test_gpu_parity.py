@@ -175,6 +175,31 @@ def test_split_launch_forms(gpu, oracle, case):
                 f"split {case} DFT comp {c}"
 
 
+def test_split_sources_at_tile_window_edges(gpu, oracle):
+    """Sources on and just outside the edges of the planner's tile windows (x
+    windows [56t-4, 56t+60), y [24t-4, 24t+28), z chunks of 16 planes with a
+    B-plane halo) and the monitored yz plane on a tile boundary: the host's
+    full/lean classification must hand every source to a full-kernel tile.
+    Nine steps: launches of 4, 4 and 1 levels (the last one classified anew)."""
+    shape = (150, 90, 70)
+    T = 9
+    eps, cb, db = iso_problem(gpu, shape, seed=9, nspheres=10, rmin=3, rmax=8)
+    cells = [(51, 19, 15), (52, 20, 16), (59, 27, 24), (60, 28, 23), (111, 43, 31),
+             (112, 44, 32), (3, 3, 8), (140, 80, 60)]
+    src = [((q % 6), *c, gpu.gaussian_pulse_table(T, 3.0, 2.0, 0.05 + 0.01 * q))
+           for q, c in enumerate(cells)]
+    setup = dict(cb=cb, db=db, sources=src, fields=random_fields(shape, 41, scale=0.05),
+                 dft=dict(comps=[0, 4], axis=0, index=56, f_cps=np.array([0.06])))
+    assert gpu.plan_tiling_for(*shape) == dict(blt=4, rows=4, waves=8, zchunk=16)
+    want, (wre, wim) = oracle_run(oracle, shape, setup, T)
+    say(f"[split edges] {shape}")
+    got, dft = run_variant(gpu, shape, setup, T, gpu.TILED, None)
+    assert_fields_equal(got, want, "split edges")
+    for n, c in enumerate(setup["dft"]["comps"]):
+        assert bits_equal(dft[c][0], wre[n]) and bits_equal(dft[c][1], wim[n]), \
+            f"split edges DFT comp {c}"
+
+
 def test_random_initial_fields_full_material(gpu, oracle):
     """Random fields (boundary included) + random full coefficient arrays."""
     shape = (41, 38, 27)
