@@ -947,10 +947,10 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     // (no source or DFT code in its loop) on the context's stream.
     const bool lean = gd::zstream_lean_supported(zt);
     const bool extras = s.n > 0 || d.ncomp > 0;
-    if (lean && extras && !c->side) {
-      GD_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-      GD_HIP(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
-      GD_HIP(c, hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+    if (lean && extras) {
+      if (!c->side) GD_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+      if (!c->fork_ev) GD_HIP(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+      if (!c->join_ev) GD_HIP(c, hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     }
     for (auto& sp : c->split) sp.blt = 0;
     // classifies the tiles of tiling t into sp and uploads both id lists on
