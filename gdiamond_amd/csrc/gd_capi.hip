@@ -263,8 +263,9 @@ hipEvent_t pool_event(gd_ctx* c, size_t i) {
 
 // Tiling planner: 4 rows x 8 waves (a 64 x 32 input tile), the fastest
 // measured shape on MI355X, with B levels per launch. The z chunk is at most
-// 64 owned planes and shrinks (down to 16) until the launch has ~1024
-// workgroups (4 per CU on 256 CUs).
+// 32 owned planes for 4- and 8-byte fields and 64 for 16-bit fields, and
+// shrinks (down to 16) until the launch has ~1024 workgroups (4 per CU on 256
+// CUs).
 gd_tiling plan_blt(size_t nx, size_t ny, size_t nz, int storage, int blt) {
   gd_tiling t;
   t.blt = blt;
@@ -274,10 +275,11 @@ gd_tiling plan_blt(size_t nx, size_t ny, size_t nz, int storage, int blt) {
   const size_t txy = ((nx + ox - 1) / ox) * ((ny + oy - 1) / oy);
   size_t zc = nz * txy / 1024;
   if (zc < 16) zc = 16;
-  if (zc > 64) zc = 64;
-  // (64 also for 16-bit fields at 1024^3: 297 against 278 Gcell/s for
-  // 128-plane chunks since the lean kernel; with the source and DFT code in
-  // every tile the longer chunks had been faster)
+  // FP32 with the lean kernel: 32-plane chunks run 0.5-1.7 % faster than 64
+  // (512^3 220-222 vs 219-221, 768^3 225 vs 223), 24 or 16 slower (211, 201);
+  // 16-bit fields: 64 (512^3 285 vs 278 at 32; 1024^3 297 vs 278 at 128)
+  const size_t zmax = is16(storage) ? 64 : 32;
+  if (zc > zmax) zc = zmax;
   if (zc > nz) zc = nz;
   t.zchunk = (int)zc;
   return t;

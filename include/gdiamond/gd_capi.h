@@ -214,7 +214,8 @@ int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
 /* The planner's tiling for a grid and storage format, without a context or
  * a device.
  * Planner rule: 4 rows x 8 waves; 3 levels for 16-bit fields, 4 otherwise;
- * z chunks of 16..64 planes sized for ~1024 workgroups. No counterpart in the reference,
+ * z chunks sized for ~1024 workgroups within 16..32 planes (16..64 for 16-bit
+ * fields). No counterpart in the reference,
  * whose tile sizes are compile-time constants (kernels_mm_ver2.cuh:8-32). */
 int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* out);
 

@@ -124,13 +124,14 @@ def test_create_without_device_or_with_bad_sizes(gd):
 
 def test_planner_without_device(gd):
     """The planner's tilings: 4 rows x 8 waves; 4 levels (FP32, FP64) or 3
-    (16-bit); z chunks sized for ~1024 workgroups within 16..64 planes."""
+    (16-bit); z chunks sized for ~1024 workgroups within 16..32 planes (4- and
+    8-byte fields) or 16..64 (16-bit)."""
     p = gd.plan_tiling_for
-    assert p(512, 512, 512) == dict(blt=4, rows=4, waves=8, zchunk=64)
+    assert p(512, 512, 512) == dict(blt=4, rows=4, waves=8, zchunk=32)
     assert p(512, 512, 512, gd.F16) == dict(blt=3, rows=4, waves=8, zchunk=64)
     assert p(512, 512, 512, gd.BF16)["blt"] == 3
     assert p(1024, 1024, 1024, gd.F16)["zchunk"] == 64
-    assert p(1024, 1024, 1024)["zchunk"] == 64
+    assert p(1024, 1024, 1024)["zchunk"] == 32
     # 128^3: 3 x 6 tiles -> 128*18/1024 = 2 planes, raised to 16
     assert p(128, 128, 128) == dict(blt=4, rows=4, waves=8, zchunk=16)
     # 256 x 256 x 8: the chunk is the whole (short) z extent
