@@ -549,6 +549,30 @@ def main():
     run_spec = spectra(ctx, gd)
     run_ms = ctx.last_run_stats()["total_ms"]
 
+    # The reference's GPU drivers take host arrays: each call copies the
+    # coefficient arrays to the device (gdiamond_gpu.cuh:878-889), steps, and
+    # copies the six fields back (gdiamond_gpu.cuh:921-926). The same round
+    # trip through the library (eps -> Cb and the material index from host
+    # memory, one run, six fields out to pageable host arrays) is timed once
+    # here and reported beside `value`, never as it.
+    host_io = None
+    if variant == gd.TILED:
+        ctx.reset()  # fields zeroed on the device, step 0, monitor sums cleared
+        t0 = time.perf_counter()
+        ctx.set_isotropic_eps(eps, dt)
+        t1 = time.perf_counter()
+        ctx.run(T, variant, tiling)
+        t2 = time.perf_counter()
+        back = ctx.get_fields()
+        t3 = time.perf_counter()
+        del back
+        host_io = {"value": round(n ** 3 * T / (t3 - t0) / 1e9, 3), "unit": "Gcell-updates/s",
+                   "in_s": round(t1 - t0, 4), "run_s": round(t2 - t1, 4),
+                   "out_s": round(t3 - t2, 4),
+                   "sample": "one run with the material taken from host eps and the six "
+                             "fields copied back to pageable host arrays (PCIe-inclusive)"}
+        print(f"# host round trip {host_io}", file=sys.stderr, flush=True)
+
     cells = n ** 3
     total = sum(wall)
     value = cells * T * args.steps / total / 1e9
@@ -591,6 +615,8 @@ def main():
                    "dft_finite": finite},
         "roofline": roofline,
     }
+    if host_io:
+        out["host_round_trip"] = host_io
     if (not args.no_error_leg and args.storage == "f32" and variant == gd.TILED and
             args.material != "indexed"):
         ctx.close()
