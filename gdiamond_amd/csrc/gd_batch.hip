@@ -578,7 +578,7 @@ int gd_batch_set_isotropic(gd_batch* b, int i, const float* cb, float db) {
   // the indexed form as well when the medium has few enough values: a
   // launch runs it when every instance has it (1 byte of material per cell
   // instead of 4; bit-identical results)
-  std::vector<uint8_t> id;
+  gd::HostVec<uint8_t> id;
   std::vector<float> table;
   const bool indexed = gd::index_materials(cb, in.N, id, table);
   if (indexed) {

@@ -549,7 +549,7 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (want_index) {
     if (c->storage == GD_STORAGE_F64)
       return c->fail(GD_ERR_UNSUPPORTED, "the indexed material needs FP32 or 16-bit storage");
-    std::vector<uint8_t> id;
+    gd::HostVec<uint8_t> id;
     std::vector<float> table;
     if (gd::index_materials(cb, c->N, id, table))
       return upload_indexed(c, id.data(), table.data(), (int)table.size(), db);
@@ -603,7 +603,7 @@ int gd_set_isotropic_indexed(gd_ctx* c, const uint8_t* id, const float* table, i
 int gd_set_isotropic_eps(gd_ctx* c, const float* eps_r, float dt) {
   if (!c) return GD_ERR_ARG;
   if (!eps_r) return c->fail(GD_ERR_ARG, "null eps array");
-  std::vector<float> cb(c->N);
+  gd::HostVec<float> cb(c->N);
   const double ddt = dt, ddx = c->dx;
   std::atomic<bool> bad{false};
   parallel_for(c->N, [&](size_t b, size_t e) {

@@ -13,7 +13,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
+#include <thread>
+#include <utility>
 #include <vector>
 
 namespace gd {
@@ -55,36 +59,113 @@ struct Scales {
 enum MatKind { kMatIso = 0, kMatFull = 1, kMatIdx = 2 };
 constexpr int kMaxMaterials = 256;
 
+// Host vector whose elements start uninitialised: large staging arrays are
+// then first touched by the threads that fill them, not zeroed on one thread.
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInitAlloc<U>;
+  };
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    if constexpr (sizeof...(A) == 0)
+      ::new ((void*)p) U;
+    else
+      ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using HostVec = std::vector<T, DefaultInitAlloc<T>>;
+
+// Ids of float bit patterns in order of first insertion, at most
+// kMaxMaterials of them (open addressing over 1024 slots, load <= 1/4).
+struct BitsIndex {
+  static constexpr unsigned kSlots = 1024;
+  uint32_t key[kSlots];
+  int val[kSlots];
+  std::vector<uint32_t> order;  // bit patterns by id
+  BitsIndex() {
+    for (unsigned q = 0; q < kSlots; ++q) val[q] = -1;
+  }
+  // id of `bits`, inserted as the next id when new; -1 when new and all
+  // kMaxMaterials ids are taken
+  int find_or_insert(uint32_t bits) {
+    unsigned h = (bits * 2654435761u) >> 22;
+    while (val[h] >= 0 && key[h] != bits) h = (h + 1) & (kSlots - 1);
+    if (val[h] < 0) {
+      if ((int)order.size() == kMaxMaterials) return -1;
+      key[h] = bits;
+      val[h] = (int)order.size();
+      order.push_back(bits);
+    }
+    return val[h];
+  }
+};
+
 // Material ids of an isotropic Cb array when it holds at most
 // kMaxMaterials distinct bit patterns (ids in order of first appearance).
 // Returns false when there are more.
-inline bool index_materials(const float* cb, size_t n, std::vector<uint8_t>& id,
+// Up to 16 host threads each number the values of one contiguous range in
+// their own first-appearance order; merging the ranges' lists in range order
+// then gives every value its global first-appearance id (a value first
+// appears in the earliest range that holds it, and within that range in its
+// local order), and the ranges whose local ids differ are renumbered.
+inline bool index_materials(const float* cb, size_t n, HostVec<uint8_t>& id,
                             std::vector<float>& table) {
-  constexpr unsigned kSlots = 1024;  // open addressing, load <= 1/4
-  uint32_t key[kSlots];
-  int val[kSlots];
-  for (unsigned q = 0; q < kSlots; ++q) val[q] = -1;
   id.resize(n);
   table.clear();
-  uint32_t last_bits = 0;
-  int last_id = -1;
-  for (size_t q = 0; q < n; ++q) {
-    uint32_t bits;
-    std::memcpy(&bits, &cb[q], 4);
-    if (last_id < 0 || bits != last_bits) {
-      unsigned h = (bits * 2654435761u) >> 22;
-      while (val[h] >= 0 && key[h] != bits) h = (h + 1) & (kSlots - 1);
-      if (val[h] < 0) {
-        if ((int)table.size() == kMaxMaterials) return false;
-        key[h] = bits;
-        val[h] = (int)table.size();
-        table.push_back(cb[q]);
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(nt, 16u));
+  nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, n >> 20));
+  const size_t chunk = (n + nt - 1) / nt;
+  std::vector<BitsIndex> loc(nt);
+  std::vector<char> over(nt, 0);
+  auto run = [&](auto&& f) {
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back([&f, t] { f(t); });
+    f(0u);
+    for (auto& x : th) x.join();
+  };
+  run([&](unsigned t) {
+    const size_t b = std::min(n, (size_t)t * chunk), e = std::min(n, b + chunk);
+    uint32_t last_bits = 0;
+    int last_id = -1;
+    for (size_t q = b; q < e; ++q) {
+      uint32_t bits;
+      std::memcpy(&bits, &cb[q], 4);
+      if (last_id < 0 || bits != last_bits) {
+        last_id = loc[t].find_or_insert(bits);
+        if (last_id < 0) {
+          over[t] = 1;
+          return;
+        }
+        last_bits = bits;
       }
-      last_bits = bits;
-      last_id = val[h];
+      id[q] = (uint8_t)last_id;
     }
-    id[q] = (uint8_t)last_id;
+  });
+  for (unsigned t = 0; t < nt; ++t)
+    if (over[t]) return false;
+  BitsIndex glob;
+  std::vector<std::vector<uint8_t>> map(nt);
+  std::vector<char> ident(nt, 1);
+  for (unsigned t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < loc[t].order.size(); ++k) {
+      const int g = glob.find_or_insert(loc[t].order[k]);
+      if (g < 0) return false;
+      map[t].push_back((uint8_t)g);
+      if ((size_t)g != k) ident[t] = 0;
+    }
   }
+  run([&](unsigned t) {
+    if (ident[t]) return;
+    const size_t b = std::min(n, (size_t)t * chunk), e = std::min(n, b + chunk);
+    const uint8_t* m = map[t].data();
+    for (size_t q = b; q < e; ++q) id[q] = m[id[q]];
+  });
+  table.resize(glob.order.size());
+  std::memcpy(table.data(), glob.order.data(), table.size() * sizeof(float));
   return true;
 }
 

@@ -67,3 +67,32 @@ def test_staged_read_after_run(gd):
         got = ctx.get_fields()
         for a in range(6):
             assert bits_equal(got[a], want[a]), a
+
+
+def test_material_from_host_eps_large_grid(gd):
+    """Material set from host eps on a grid that the host indexes on several
+    threads (6.3 M cells): the indexed form holds one entry per distinct Cb
+    bit pattern and steps to the same fields as the dense form."""
+    shape = (256, 256, 96)
+    nx, ny, nz = shape
+    eps = gd.sphere_medium(nx, ny, nz, 21, 40, 3.0, 12.0, 1.0, 12.0)
+    dt = np.float32(0.99 / np.sqrt(3.0))
+    cb, _ = gd.iso_cb_from_eps(eps, dt, 1.0)
+    distinct = np.unique(cb.view(np.uint32)).size
+    assert 2 < distinct <= 256
+    src = gd.gaussian_pulse_table(8, 3.0, 2.0, 0.05)
+    out = {}
+    for fmt in (gd.MAT_INDEXED, gd.MAT_DENSE):
+        with gd.Context(*shape) as ctx:
+            ctx.set_material_format(fmt)
+            ctx.set_isotropic_eps(eps, dt)
+            f, ntable = ctx.get_material_format()
+            assert f == fmt
+            if fmt == gd.MAT_INDEXED:
+                assert ntable == distinct
+            ctx.add_source(gd.EZ, nx // 2, ny // 2, nz // 2, src)
+            ctx.set_fields(_fields(shape, 13))
+            ctx.run(8, gd.TILED)
+            out[fmt] = ctx.get_fields()
+    for a in range(6):
+        assert bits_equal(out[gd.MAT_INDEXED][a], out[gd.MAT_DENSE][a]), a
