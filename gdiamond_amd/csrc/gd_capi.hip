@@ -177,8 +177,7 @@ void parallel_for(size_t n, F&& f) {
 constexpr size_t kStageBytes = size_t(64) << 20;
 constexpr size_t kStageMin = size_t(8) << 20;
 
-template <class Take>
-int download(gd_ctx* c, const void* src, size_t bytes, Take&& take) {
+int ensure_staging(gd_ctx* c) {
   for (int k = 0; k < 2; ++k) {
     if (!c->stage[k]) {
       hipError_t e = hipHostMalloc(&c->stage[k], kStageBytes, hipHostMallocDefault);
@@ -195,6 +194,13 @@ int download(gd_ctx* c, const void* src, size_t bytes, Take&& take) {
       }
     }
   }
+  return GD_OK;
+}
+
+template <class Take>
+int download(gd_ctx* c, const void* src, size_t bytes, Take&& take) {
+  const int rc = ensure_staging(c);
+  if (rc != GD_OK) return rc;
   const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
   const unsigned char* s = static_cast<const unsigned char*>(src);
   // chunk i goes to slot i & 1; slot (i + 1) & 1 is free again once the host
@@ -213,6 +219,34 @@ int download(gd_ctx* c, const void* src, size_t bytes, Take&& take) {
     const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
     take(static_cast<const unsigned char*>(c->stage[i & 1]), off, len);
   }
+  return GD_OK;
+}
+
+// Host -> device copy of `bytes` from host memory: chunks of 8 MiB and more
+// go through the two pinned staging slots (the host threads fill one while
+// the DMA of the other runs), smaller copies go direct. Returns after the
+// data is on the device.
+int copy_in(gd_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes < kStageMin) {
+    GD_HIP(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GD_OK;
+  }
+  const int rc = ensure_staging(c);
+  if (rc != GD_OK) return rc;
+  const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+  const unsigned char* h = static_cast<const unsigned char*>(src);
+  unsigned char* d = static_cast<unsigned char*>(dst);
+  for (size_t i = 0; i < nch; ++i) {
+    const int k = (int)(i & 1);
+    // slot k was last read by the DMA of chunk i - 2
+    if (i >= 2) GD_HIP(c, hipEventSynchronize(c->stage_ev[k]));
+    const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    unsigned char* st = static_cast<unsigned char*>(c->stage[k]);
+    parallel_for(len, [&](size_t b, size_t e) { std::memcpy(st + b, h + off + b, e - b); });
+    GD_HIP(c, hipMemcpyAsync(d + off, st, len, hipMemcpyHostToDevice, c->stream));
+    GD_HIP(c, hipEventRecord(c->stage_ev[k], c->stream));
+  }
+  GD_HIP(c, hipStreamSynchronize(c->stream));
   return GD_OK;
 }
 
@@ -289,7 +323,8 @@ int upload_indexed(gd_ctx* c, const uint8_t* id, const float* table, int ntable,
   }
   float padded[GD_MAX_MATERIALS] = {0.0f};
   std::memcpy(padded, table, (size_t)ntable * sizeof(float));
-  GD_HIP(c, hipMemcpy(c->mat_id, id, c->N, hipMemcpyHostToDevice));
+  const int rc = copy_in(c, c->mat_id, id, c->N);
+  if (rc != GD_OK) return rc;
   GD_HIP(c, hipMemcpy(c->mat_table, padded, sizeof(padded), hipMemcpyHostToDevice));
   c->ntable = ntable;
   c->db = db;
@@ -504,9 +539,11 @@ int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
     }
     if (c->storage == GD_STORAGE_F64) {
       tmp.assign(coeff[a], coeff[a] + c->N);
-      GD_HIP(c, hipMemcpy(c->coeff[a], tmp.data(), c->N * es, hipMemcpyHostToDevice));
+      const int rc = copy_in(c, c->coeff[a], tmp.data(), c->N * es);
+      if (rc != GD_OK) return rc;
     } else {
-      GD_HIP(c, hipMemcpy(c->coeff[a], coeff[a], c->N * es, hipMemcpyHostToDevice));
+      const int rc = copy_in(c, c->coeff[a], coeff[a], c->N * es);
+      if (rc != GD_OK) return rc;
     }
   }
   c->mat_kind = gd::kMatFull;
@@ -568,9 +605,11 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
     parallel_for(c->N, [&](size_t b, size_t e) {
       for (size_t q = b; q < e; ++q) buf[q] = (double)cb[q];
     });
-    GD_HIP(c, hipMemcpy(c->cb, buf.data(), c->N * ces, hipMemcpyHostToDevice));
+    const int rc = copy_in(c, c->cb, buf.data(), c->N * ces);
+    if (rc != GD_OK) return rc;
   } else {
-    GD_HIP(c, hipMemcpy(c->cb, cb, c->N * ces, hipMemcpyHostToDevice));
+    const int rc = copy_in(c, c->cb, cb, c->N * ces);
+    if (rc != GD_OK) return rc;
   }
   c->db = db;
   c->mat_kind = gd::kMatIso;
@@ -664,7 +703,8 @@ int gd_set_fields(gd_ctx* c, const float* const fields[6]) {
   std::vector<unsigned char> buf;
   for (int a = 0; a < 6; ++a) {
     encode_component(c, a, fields[a], buf);
-    GD_HIP(c, hipMemcpy(c->f[c->cur][a], buf.data(), c->N * c->esz, hipMemcpyHostToDevice));
+    const int rc = copy_in(c, c->f[c->cur][a], buf.data(), c->N * c->esz);
+    if (rc != GD_OK) return rc;
   }
   return GD_OK;
 }

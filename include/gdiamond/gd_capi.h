@@ -172,10 +172,10 @@ int gd_clear_sources(gd_ctx* ctx);
 /* ---- fields ---------------------------------------------------------------- */
 
 int gd_set_fields(gd_ctx* ctx, const float* const fields[6]);
-/* Field reads of 8 MiB and more per component go through two 64 MiB pinned
- * host buffers that the context allocates on its first such read and
- * gd_destroy frees (the reference's drivers copy the fields back on every
- * call, gdiamond_gpu.cuh:921-926). */
+/* Field and material copies of 8 MiB and more per array go through two
+ * 64 MiB pinned host buffers that the context allocates on its first such
+ * copy and gd_destroy frees (the reference's drivers copy the coefficients in
+ * and the fields back on every call, gdiamond_gpu.cuh:878-889, :921-926). */
 int gd_get_fields(gd_ctx* ctx, float* const fields[6]);
 /* Exact field values in double for every storage format. */
 int gd_get_fields_f64(gd_ctx* ctx, double* const fields[6]);
