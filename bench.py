@@ -28,13 +28,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic bytes per cell-update: 6 fields x (read + write) + the material
-# (a 1-byte material id for the indexed form, one FP32 Cb for the dense form)
+# algorithmic bytes per cell-update, one fixed basis per field format: 6 fields
+# x (read + write) + one per-cell material coefficient of the field's width
+# (FP32: 48 + 4 = 52 B; binary16: 24 + 2 = 26 B). The 1-byte material id form
+# moves 3 (FP32) or 1 (binary16) bytes less; the basis does not follow it, so
+# dense and indexed runs stay comparable.
 FIELD_BYTES = {"f32": 4, "f16": 2}
 
 
-def bytes_alg(storage, material):
-    return 6 * FIELD_BYTES[storage] * 2 + (1 if material == "indexed" else 4)
+def bytes_alg(storage):
+    return 6 * FIELD_BYTES[storage] * 2 + FIELD_BYTES[storage]
 
 
 F16_SCALE_EXP = -8  # binary16 range of the stored fields: ~2.4e-7 .. 256
@@ -43,19 +46,18 @@ HBM_PEAK_GBS = 8000.0
 # stream at the same time, the full kernel over the tiles with the source or
 # the monitored plane; launch times span both
 TILED_KERNELS = "zstream_lean_kernel + zstream_kernel (concurrent pair per launch)"
+PLAIN_KERNELS = "plain_e_kernel + plain_h_kernel (one pair per step)"
 # the headline metric of the 512^3 workload
 METRIC = "Gcell-updates/s at 512\u00b3 + % HBM roofline; chosen vs plain speedup at fixed error"
 
 
-def parse_traffic(path, tiling, n, storage, material):
-    """HBM bytes per launch of the stepping kernel from a committed rocprofv3
-    PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
-    calibration). Used only when it was measured for this tiling, size,
-    storage and material."""
-    if not os.path.exists(path) or n != 512:
+def parse_traffic(path, want):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 PMC summary
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration). Used only
+    when the summary holds every line of `want` (the tiling, size, storage and
+    material it was measured for)."""
+    if not os.path.exists(path):
         return None
-    want = {"tiling: " + " ".join(f"{k}={tiling[k]}" for k in ("blt", "rows", "waves", "zchunk")),
-            f"storage: {storage}", f"material: {material}"}
     seen, val = set(), None
     with open(path) as f:
         for ln in f:
@@ -63,7 +65,21 @@ def parse_traffic(path, tiling, n, storage, material):
                 seen.add(ln.strip())
             if ln.startswith("bytes_per_launch:"):
                 val = float(ln.split(":", 1)[1])
-    return val if seen == want else None
+    return val if seen == set(want) else None
+
+
+def tiling_line(tiling):
+    return "tiling: " + " ".join(f"{k}={tiling[k]}" for k in ("blt", "rows", "waves", "zchunk"))
+
+
+def measured_hbm(traffic, launches, kernel_ms, cell_updates):
+    """Real HBM use from the measured bytes per launch: bytes of all launches
+    of a run over their summed duration, and bytes per cell-update."""
+    if traffic is None or not kernel_ms:
+        return None
+    gbs = traffic * launches / (kernel_ms * 1e-3) / 1e9
+    return {"GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_cell_update": round(traffic * launches / cell_updates, 2)}
 
 
 def build_problem(gd, n, timesteps, seed=42):
@@ -89,23 +105,6 @@ def cpu_baseline(gd, n, eps, dt, src, steps, threads):
           threads=threads)
     dt_s = time.perf_counter() - t0
     return n ** 3 * steps / dt_s / 1e9
-
-
-def reference_seq_rate(n=256, steps=4):
-    """The reference's own update_FDTD_seq (gdiamond.hpp:1164-1253), compiled
-    from its sources into oracle/_ref, on its default problem: the Mcells/s it
-    measures and prints itself, as Gcell-updates/s. None when that build is
-    absent."""
-    import ctypes as C
-    from oracle import oracle as O
-    try:
-        L = O.ref_lib()
-    except (OSError, FileNotFoundError):
-        return None
-    v = C.c_double(0.0)
-    if L.ref_seq_rate(n, n, n, steps, C.byref(v)) != 0:
-        return None
-    return v.value / 1e3
 
 
 def host_cpu():
@@ -181,13 +180,12 @@ def error_leg(gd, n, T, eps, dt, src, f_cps, tiled_spec, tiled_ms):
             "speedup_chosen_vs_f32_tiled": speedup("f32-tiled")}
 
 
-def run_precision(args, gd):
+def run_precision(gd, n=1024, steps=2, warmup=1, T=2000, error_leg=True):
     """Precision form: FP16 field storage (scale 2^F16_SCALE_EXP, exact FP32 Cb
     per cell), FP32 arithmetic, FP64 DFT, TILED kernel.
-    Throughput on the 1024^3 sphere medium (seed 43) for 2000 steps; error of
+    Throughput on the n^3 sphere medium (seed 43) for 2000 steps; error of
     the FP16, BF16 and FP32 forms against an FP64 run of the identical problem
-    at 256^3 (same generator and seed, 2000 steps)."""
-    T = 2000
+    at 256^3 (same generator and seed, 2000 steps). Returns the record."""
     dt = np.float32(0.99 / math.sqrt(3.0))
     src = gd.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
     f_cps = np.array([0.03, 0.05, 0.07])
@@ -211,9 +209,10 @@ def run_precision(args, gd):
     eps_s = gd.sphere_medium(ne, ne, ne, 43, 200, 4.0, 20.0, 1.0, 12.0)
     res = {}
     ref = None
-    for name, storage, variant in (("f64-plain", gd.F64, gd.PLAIN), ("f32-tiled", gd.F32, gd.TILED),
-                                   ("f16-tiled", gd.F16, gd.TILED), ("f16-plain", gd.F16, gd.PLAIN),
-                                   ("bf16-tiled", gd.BF16, gd.TILED)):
+    forms = (("f64-plain", gd.F64, gd.PLAIN), ("f32-tiled", gd.F32, gd.TILED),
+             ("f16-tiled", gd.F16, gd.TILED), ("f16-plain", gd.F16, gd.PLAIN),
+             ("bf16-tiled", gd.BF16, gd.TILED))
+    for name, storage, variant in (forms if error_leg else ()):
         ctx = make(ne, storage, eps_s)
         ctx.run(T, variant)
         spec = spectra(ctx, gd)
@@ -226,53 +225,61 @@ def run_precision(args, gd):
         print(f"# 256^3 {name}: {res[name]}", file=sys.stderr, flush=True)
     del eps_s
 
-    # throughput at 1024^3
-    n = args.size if args.size != 512 else 1024
+    # throughput at n^3
     t0 = time.perf_counter()
     eps = gd.sphere_medium(n, n, n, 43, 200, 4.0, 20.0, 1.0, 12.0)
     ctx = make(n, gd.F16, eps)
     del eps
     fmt, ntable = ctx.get_material_format()
+    material = "indexed" if fmt == gd.MAT_INDEXED else "dense"
     ctx.set_launch_timing(True)
     tiling = ctx.plan_tiling()
     print(f"# {n}^3 setup {time.perf_counter() - t0:.1f}s tiling={tiling} material={fmt},{ntable}",
           file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctx.reset()
         ctx.run(T, gd.TILED)
     ms, kms = [], []
-    for _ in range(args.steps):
+    nlaunch = 0
+    for _ in range(steps):
         ctx.reset()
         ctx.run(T, gd.TILED)
         st = ctx.last_run_stats()
         ms.append(st["total_ms"])
         kms.append(st["kernel_ms"])
+        nlaunch = st["nlaunch"]
         print(f"# run {st}", file=sys.stderr, flush=True)
     re, im = ctx.dft_get(gd.EZ)
     finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
     ctx.close()
     cells = float(n) ** 3
     value = cells * T / (np.mean(ms) * 1e-3) / 1e9
-    b_alg = bytes_alg("f16", "indexed" if fmt == gd.MAT_INDEXED else "dense")
+    b_alg = bytes_alg("f16")
     achieved = b_alg * cells * T / (np.mean(kms) * 1e-3) / 1e9
-    out = {
-        "metric": "Gcell-updates/s at 1024^3, FP16 field storage",
-        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(float(np.mean(ms)), 2),
+    traffic_name = f"zstream_{n}cube_f16_{material}_traffic.txt"
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", traffic_name),
+                            {tiling_line(tiling), "storage: f16", f"material: {material}",
+                             f"grid: {n}^3"})
+    return {
+        "metric": f"Gcell-updates/s at {n}^3, FP16 field storage",
+        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(float(np.mean(ms)), 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16",
         "data": "synthetic",
         "config": {"workload": f"fdtd3d-yee-spheres-{n}^3-{T}steps-f16", "grid": [n, n, n],
                    "timesteps": T, "medium": "200 spheres seed 43, eps 1-12",
                    "storage": f"binary16 x 2^{F16_SCALE_EXP}", "tiling": tiling,
-                   "material": "indexed" if fmt == gd.MAT_INDEXED else "dense",
-                   "dft_finite": finite,
+                   "material": material, "dft_finite": finite,
                    "error_256cube": {"reference": "f64-plain", "forms": res,
                                      "bound": ERROR_BOUND}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": TILED_KERNELS, "bytes_alg_per_cell_update": b_alg},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": TILED_KERNELS, "bytes_alg_per_cell_update": b_alg,
+                     "mean_launch_ms": round(float(np.mean(kms)) / max(1, nlaunch), 4),
+                     "launches_per_run": nlaunch,
+                     "measured": measured_hbm(traffic, nlaunch, float(np.mean(kms)), cells * T),
+                     "traffic_source": "profiles/" + traffic_name if traffic else None},
     }
-    print(json.dumps(out), flush=True)
 
 
 def run_learned(args, gd):
@@ -354,20 +361,20 @@ def run_learned(args, gd):
     print(json.dumps(out), flush=True)
 
 
-def run_batch(args, gd):
+def run_batch(gd, batch_size=256, T=500, steps=2, warmup=1,
+              modes=("graph", "persistent", "per_instance")):
     """Batched workload: `batch_size` independent FP32 problems, cube edge drawn
     uniformly from {32,48,64,96,128,160} (numpy default_rng(2024)), each with
     its own sphere medium (gd_sphere_medium seed 1000+i, 200 spheres, radius
     U[4,20], eps U[1,12]), a Gaussian Jz point source at its centre with its
-    own frequency U[0.02,0.08] cycles/step, `batch_timesteps` steps. Tilings
-    per size class from the autotuner. One step of the benchmark = one full
-    run of every instance; value = total cell-updates / mean run time (device
-    events around the run, fields reset outside)."""
-    T = args.batch_timesteps
+    own frequency U[0.02,0.08] cycles/step, T steps. Tilings per size class
+    from the autotuner. One step of the benchmark = one full run of every
+    instance; value = total cell-updates / mean run time (device events around
+    the run, fields reset outside). Returns the record."""
     dt = np.float32(0.99 / math.sqrt(3.0))
     rng = np.random.default_rng(2024)
-    sizes = rng.choice([32, 48, 64, 96, 128, 160], size=args.batch_size)
-    freqs = rng.uniform(0.02, 0.08, size=args.batch_size)
+    sizes = rng.choice([32, 48, 64, 96, 128, 160], size=batch_size)
+    freqs = rng.uniform(0.02, 0.08, size=batch_size)
     dims = [(int(s), int(s), int(s)) for s in sizes]
     t0 = time.perf_counter()
     b = gd.Batch(dims, dx=1.0)
@@ -386,22 +393,24 @@ def run_batch(args, gd):
     tune_s = time.perf_counter() - t0
     print(f"# autotune {tune_s:.1f}s: {classes}", file=sys.stderr, flush=True)
 
-    def timed(mode, steps, warmup):
-        for _ in range(warmup):
+    def timed(mode, nrun, nwarm):
+        for _ in range(nwarm):
             b.reset()
             b.run(T, mode)
         ms = []
-        for _ in range(steps):
+        for _ in range(nrun):
             b.reset()
             b.run(T, mode)
             ms.append(b.last_run_stats()["total_ms"])
         return float(np.mean(ms)), b.last_run_stats()["nlaunch"]
 
     res = {}
-    for name, mode, steps in (("graph", gd.BATCH_GRAPH, args.steps),
-                              ("persistent", gd.BATCH_PERSISTENT, 1),
-                              ("per_instance", gd.BATCH_PER_INSTANCE, 1)):
-        ms, nl = timed(mode, steps, args.warmup)
+    for name, mode, nrun in (("graph", gd.BATCH_GRAPH, steps),
+                             ("persistent", gd.BATCH_PERSISTENT, 1),
+                             ("per_instance", gd.BATCH_PER_INSTANCE, 1)):
+        if name not in modes and name != "graph":
+            continue
+        ms, nl = timed(mode, nrun, warmup)
         res[name] = {"ms_per_run": round(ms, 2), "GCUPS": round(cells * T / (ms * 1e-3) / 1e9, 2),
                      "launches_per_run": nl}
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
@@ -409,33 +418,47 @@ def run_batch(args, gd):
     b.run(T, gd.BATCH_GRAPH)
     fin = b.get_fields(0)
     finite = bool(all(np.isfinite(x).all() for x in fin))
+    b.close()
     g = res["graph"]
-    b_alg = bytes_alg("f32", "indexed" if indexed else "dense")
+    b_alg = bytes_alg("f32")
     achieved = b_alg * cells * T / (g["ms_per_run"] * 1e-3) / 1e9
-    out = {
+    material = "indexed" if indexed else "dense"
+    # the persistent launches of one run: per launch group, one launch per
+    # blt steps (a full-kernel and a lean-kernel dispatch), plus the advance
+    nlaunch = g["launches_per_run"] - 1
+    traffic_name = f"zstream_batch_{batch_size}inst_traffic.txt"
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", traffic_name),
+                            {f"instances: {batch_size}", f"timesteps: {T}",
+                             f"material: {material}"})
+    return {
         "metric": "Gcell-updates/s, batched independent problems",
         "value": g["GCUPS"],
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": g["ms_per_run"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"fdtd3d-yee-batch-{args.batch_size}x-{T}steps",
+        "config": {"workload": f"fdtd3d-yee-batch-{batch_size}x-{T}steps",
                    "sizes": "cube edge from {32,48,64,96,128,160}, default_rng(2024)",
-                   "cells": cells, "timesteps": T, "classes": classes,
+                   "cells": cells, "timesteps": T, "classes": classes, "material": material,
                    "autotune_s": round(tune_s, 1), "modes": res, "finite": finite,
-                   "speedup_graph_vs_per_instance": round(res["per_instance"]["ms_per_run"] /
-                                                          g["ms_per_run"], 3)},
+                   "speedup_graph_vs_per_instance":
+                       round(res["per_instance"]["ms_per_run"] / g["ms_per_run"], 3)
+                       if "per_instance" in res else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "zstream_batch_kernel", "bytes_alg_per_cell_update": b_alg},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "zstream_batch_kernel (full, then lean, per launch)",
+                     "bytes_alg_per_cell_update": b_alg,
+                     "mean_launch_ms": round(g["ms_per_run"] / max(1, nlaunch), 4),
+                     "launches_per_run": nlaunch,
+                     "measured": measured_hbm(traffic, nlaunch, g["ms_per_run"], cells * T),
+                     "traffic_source": "profiles/" + traffic_name if traffic else None},
     }
-    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -463,12 +486,17 @@ def main():
                     default="single",
                     help="single: the 512^3 problem; batch: 256 independent problems; "
                          "precision: FP16 storage at 1024^3 with its error at 256^3")
+    ap.add_argument("--no-legs", dest="legs", action="store_false",
+                    help="skip the precision (1024^3 FP16) and batch legs of the default run")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--learned-pairs", type=int, default=64)
     ap.add_argument("--learned-epochs", type=int, default=6)
     ap.add_argument("--learned-params-out", default="",
                     help="write the trained corrector parameters to this .npz file")
     ap.add_argument("--batch-timesteps", type=int, default=500)
+    ap.add_argument("--batch-modes", default="graph,persistent,per_instance",
+                    help="batch run modes to time (comma-separated; graph always runs)")
+    ap.add_argument("--precision-timesteps", type=int, default=2000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the copy kernel used to calibrate HBM counters")
     args = ap.parse_args()
@@ -481,10 +509,13 @@ def main():
     import gdiamond_amd as gd
 
     if args.workload == "batch":
-        run_batch(args, gd)
+        print(json.dumps(run_batch(gd, args.batch_size, args.batch_timesteps, args.steps,
+                                   args.warmup, tuple(args.batch_modes.split(",")))), flush=True)
         return
     if args.workload == "precision":
-        run_precision(args, gd)
+        n = args.size if args.size != 512 else 1024
+        print(json.dumps(run_precision(gd, n, args.steps, args.warmup, args.precision_timesteps,
+                                       not args.no_error_leg)), flush=True)
         return
     if args.workload == "learned":
         run_learned(args, gd)
@@ -578,19 +609,27 @@ def main():
     value = cells * T * args.steps / total / 1e9
     ms_per_step = total / args.steps * 1e3
     mean_launch_ms = sum(kernel_ms) / max(1, nlaunch * args.steps)
-    # algorithmic bytes of all launches of one run / their summed duration
-    b_alg = bytes_alg(args.storage, material)
+    # algorithmic bytes of all launches of one run / their summed duration;
+    # with B steps per TILED launch this is an equivalent bandwidth and can
+    # exceed the peak. `measured` is the real HBM use from the committed
+    # FETCH_SIZE / WRITE_SIZE summary of the same launch.
+    b_alg = bytes_alg(args.storage)
     achieved = b_alg * cells * T / (np.mean(kernel_ms) * 1e-3) / 1e9
-    traffic_name = f"zstream_512cube_{args.storage}_{material}_traffic.txt"
-    traffic_file = os.path.join(ROOT, "profiles", traffic_name)
-    traffic = (parse_traffic(traffic_file, tiling, n, args.storage, material)
-               if variant == gd.TILED else None)
+    if variant == gd.TILED:
+        traffic_name = f"zstream_{n}cube_{args.storage}_{material}_traffic.txt"
+        want = {tiling_line(tiling), f"storage: {args.storage}", f"material: {material}",
+                f"grid: {n}^3"}
+    else:
+        traffic_name = f"plain_{n}cube_{args.storage}_{material}_traffic.txt"
+        want = {f"storage: {args.storage}", f"material: {material}", f"grid: {n}^3"}
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", traffic_name), want)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": TILED_KERNELS if variant == gd.TILED else "plain_e/h_kernel",
+                "kernel": TILED_KERNELS if variant == gd.TILED else PLAIN_KERNELS,
                 "bytes_alg_per_cell_update": b_alg,
                 "mean_launch_ms": round(mean_launch_ms, 4),
                 "launches_per_run": nlaunch,
+                "measured": measured_hbm(traffic, nlaunch, float(np.mean(kernel_ms)), cells * T),
                 "traffic_source": "profiles/" + traffic_name if traffic else None}
 
     out = {
@@ -635,14 +674,13 @@ def main():
         out["cpu_baseline"]["single_thread"] = {
             "value": round(seq, 4), "unit": "Gcell-updates/s",
             "sample": f"steps 1..{seq_steps}, one thread (the update_FDTD_seq loop)"}
-        ref = reference_seq_rate()
-        if ref is not None:
-            out["cpu_baseline"]["reference_seq"] = {
-                "value": round(ref, 4), "unit": "Gcell-updates/s", "kind": "reference",
-                "cores": 1,
-                "sample": "the reference's own update_FDTD_seq built from its sources "
-                          "(oracle/_ref), its default problem at 256^3, 4 steps, the rate "
-                          "it prints itself"}
+    # the precision (1024^3 FP16 storage) and batch (256 instances) workloads
+    # measured in the same default run, each with its own roofline
+    if (args.legs and args.workload == "single" and n == 512 and args.storage == "f32" and
+            variant == gd.TILED):
+        ctx.close()
+        out["precision"] = run_precision(gd, 1024, steps=2, warmup=1)
+        out["batch"] = run_batch(gd, 256, 500, steps=2, warmup=1)
     print(json.dumps(out), flush=True)
 
 

@@ -66,7 +66,7 @@ struct gd_ctx {
   double* dft_im = nullptr;
   double* dft_w = nullptr;
   size_t dft_w_cap = 0;
-  float* dft_snap = nullptr;  // TILED: [level][component][plane] plane values
+  float* dft_snap = nullptr;  // TILED: [level][slot][plane] plane values
   std::vector<double> dft_w_host;
 
   // TILED launches split in two: tiles with sources or monitored cells run
@@ -236,26 +236,35 @@ int copy_in(gd_ctx* c, void* dst, const void* src, size_t bytes) {
   const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
   const unsigned char* h = static_cast<const unsigned char*>(src);
   unsigned char* d = static_cast<unsigned char*>(dst);
+  // on an error the copies already issued are drained before returning, so
+  // that no DMA still reads a staging slot that a later transfer refills
+  auto fail = [&](hipError_t e, const char* what) {
+    (void)hipStreamSynchronize(c->stream);
+    return c->hip_fail(e, what);
+  };
   for (size_t i = 0; i < nch; ++i) {
     const int k = (int)(i & 1);
     // slot k was last read by the DMA of chunk i - 2
-    if (i >= 2) GD_HIP(c, hipEventSynchronize(c->stage_ev[k]));
+    if (i >= 2) {
+      const hipError_t e = hipEventSynchronize(c->stage_ev[k]);
+      if (e != hipSuccess) return fail(e, "hipEventSynchronize(staging)");
+    }
     const size_t off = i * kStageBytes, len = std::min(kStageBytes, bytes - off);
     unsigned char* st = static_cast<unsigned char*>(c->stage[k]);
     parallel_for(len, [&](size_t b, size_t e) { std::memcpy(st + b, h + off + b, e - b); });
-    GD_HIP(c, hipMemcpyAsync(d + off, st, len, hipMemcpyHostToDevice, c->stream));
-    GD_HIP(c, hipEventRecord(c->stage_ev[k], c->stream));
+    hipError_t e = hipMemcpyAsync(d + off, st, len, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->stage_ev[k], c->stream);
+    if (e != hipSuccess) return fail(e, "hipMemcpyAsync(staging)");
   }
   GD_HIP(c, hipStreamSynchronize(c->stream));
   return GD_OK;
 }
 
-// Host values -> storage bytes of one component (fp32 host values).
+// Host values -> storage bytes of one component (fp32 host values) for the
+// FP64 and 16-bit formats (FP32 values are uploaded as they are).
 void encode_component(const gd_ctx* c, int comp, const float* in, std::vector<unsigned char>& out) {
   out.resize(c->N * c->esz);
-  if (c->storage == GD_STORAGE_F32) {
-    std::memcpy(out.data(), in, c->N * sizeof(float));
-  } else if (c->storage == GD_STORAGE_F64) {
+  if (c->storage == GD_STORAGE_F64) {
     double* o = reinterpret_cast<double*>(out.data());
     parallel_for(c->N, [&](size_t b, size_t e) {
       for (size_t q = b; q < e; ++q) o[q] = (double)in[q];
@@ -382,6 +391,16 @@ gd_tiling plan_dims(size_t nx, size_t ny, size_t nz, int storage) {
 gd_tiling plan(const gd_ctx* c) { return plan_dims(c->nx, c->ny, c->nz, c->storage); }
 
 }  // namespace
+
+namespace gd {
+// DFT weights of absolute step n: cos(2 pi f n)*scale and sin(2 pi f n)*scale
+// in double, as every run evaluates them.
+void dft_weight(double f_cps, long long n, double scale, double* wc, double* ws) {
+  const double ang = 2.0 * 3.14159265358979323846 * f_cps * (double)n;
+  *wc = weight_cos(ang) * scale;
+  *ws = weight_sin(ang) * scale;
+}
+}  // namespace gd
 
 extern "C" {
 
@@ -702,8 +721,13 @@ int gd_set_fields(gd_ctx* c, const float* const fields[6]) {
   GD_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<unsigned char> buf;
   for (int a = 0; a < 6; ++a) {
-    encode_component(c, a, fields[a], buf);
-    const int rc = copy_in(c, c->f[c->cur][a], buf.data(), c->N * c->esz);
+    // FP32 fields are the storage bytes themselves: no host-side conversion
+    const void* src = fields[a];
+    if (c->storage != GD_STORAGE_F32) {
+      encode_component(c, a, fields[a], buf);
+      src = buf.data();
+    }
+    const int rc = copy_in(c, c->f[c->cur][a], src, c->N * c->esz);
     if (rc != GD_OK) return rc;
   }
   return GD_OK;
@@ -926,7 +950,7 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
     if (c->coeff[a]) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
   if (c->cb) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
   if (c->mat_id) b += c->N + GD_MAX_MATERIALS * sizeof(float);
-  if (c->dft_snap) b += (size_t)gd::kMaxBlt * 6 * c->dft_plane * sizeof(float);
+  if (c->dft_snap) b += (size_t)gd::kMaxBlt * c->dft_ncomp * c->dft_plane * sizeof(float);
   for (auto& s : c->src) b += s.len * sizeof(float);
   b += 2 * (size_t)c->dft_ncomp * c->dft_f.size() * c->dft_plane * sizeof(double);
   b += c->dft_w_cap * sizeof(double);
@@ -965,7 +989,7 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     int rc = alloc_fields(c, 1);
     if (rc != GD_OK) return rc;
     if (c->dft_ncomp && !c->dft_snap) {
-      const size_t n = (size_t)gd::kMaxBlt * 6 * c->dft_plane;
+      const size_t n = (size_t)gd::kMaxBlt * c->dft_ncomp * c->dft_plane;
       hipError_t e = hipMalloc(&c->dft_snap, n * sizeof(float));
       if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(dft snapshot)");
     }
@@ -1029,13 +1053,10 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     // w = cos(2 pi f n)*scale, sin(2 pi f n)*scale in double (the reference
     // evaluates cos(omega*t) per cell on the device, kernels.cuh:109-110).
     c->dft_w_host.resize(need);
-    const double two_pi = 2.0 * 3.14159265358979323846;
     for (size_t q = 0; q < nf; ++q)
-      for (size_t r = 0; r < nsteps; ++r) {
-        const double ang = two_pi * c->dft_f[q] * (double)(c->step + r);
-        c->dft_w_host[q * nsteps + r] = weight_cos(ang) * c->dft_scale;
-        c->dft_w_host[nf * nsteps + q * nsteps + r] = weight_sin(ang) * c->dft_scale;
-      }
+      for (size_t r = 0; r < nsteps; ++r)
+        gd::dft_weight(c->dft_f[q], (long long)(c->step + r), c->dft_scale,
+                       &c->dft_w_host[q * nsteps + r], &c->dft_w_host[nf * nsteps + q * nsteps + r]);
     GD_HIP(c, hipMemcpyAsync(c->dft_w, c->dft_w_host.data(), need * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
     d.mask = 0u;

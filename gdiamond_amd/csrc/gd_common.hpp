@@ -269,7 +269,7 @@ struct Dft {
   const double* ws;
   double* re;        // [slot][f][plane]
   double* im;
-  float* snap;       // TILED: plane values per level, [level][component][plane]
+  float* snap;       // TILED: plane values per level, [level][slot][plane]
 };
 
 struct Grid {

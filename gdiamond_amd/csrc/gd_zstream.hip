@@ -427,13 +427,15 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           if (okE && ((a.d.mask >> c) & 1u)) {
-            float* sp = a.d.snap + ((size_t)(b - 1) * 6 + c) * (size_t)a.d.plane;
+            const int slot = __builtin_popcount(a.d.mask & ((1u << c) - 1u));
+            float* sp = a.d.snap + ((size_t)(b - 1) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
               if (dft_row(r)) sp[dft_base(r) + zE * dft_zmul] = en[r][c];
           }
           if (okH && ((a.d.mask >> (3 + c)) & 1u)) {
-            float* sp = a.d.snap + ((size_t)(b - 1) * 6 + 3 + c) * (size_t)a.d.plane;
+            const int slot = __builtin_popcount(a.d.mask & ((1u << (3 + c)) - 1u));
+            float* sp = a.d.snap + ((size_t)(b - 1) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
               if (dft_row(r)) sp[dft_base(r) + zH * dft_zmul] = hn[r][c];
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(256) void dft_snapshot_kernel(Dft d, int nlev, int 
       const long long o = ((long long)slot * d.nf + f) * d.plane + p;
       double re = d.re[o], im = d.im[o];
       for (int b = 0; b < nlev; ++b) {
-        const double v = (double)d.snap[((long long)b * 6 + c) * d.plane + p];
+        const double v = (double)d.snap[((long long)b * d.ncomp + slot) * d.plane + p];
         re = re + v * d.wc[(long long)f * d.run_len + run_step0 + b];
         im = im - v * d.ws[(long long)f * d.run_len + run_step0 + b];
       }

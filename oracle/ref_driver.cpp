@@ -1,8 +1,10 @@
 // Thin C entry points over the reference's own CPU path, compiled from the
 // reference headers where they lie (see oracle/Makefile, target `ref`).
-// TEST INFRASTRUCTURE ONLY: used in this container to write golden vectors
-// and to cross-check the oracle restatement. Nothing here is shipped or run
-// by the product.
+// TEST INFRASTRUCTURE ONLY, used where the reference sources are present
+// (never on a GPU host: oracle/_ref is neither committed nor copied there):
+// it writes the golden vectors under tests/golden/ and cross-checks the
+// oracle restatement in the CPU tests. The product, the GPU tests and the
+// benchmark never load it.
 //
 // The reference keeps its arrays private (gdiamond.hpp:798-891); the standard
 // headers are included first and `private` is widened only for the one

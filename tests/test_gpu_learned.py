@@ -54,6 +54,49 @@ def _coarse_problem(gd, learned, n=32, seed=11):
     return prob
 
 
+def _corrected_vs_emulation(gd, learned, nc, eps_c, dt_c, jc, params, cfg, T, interval, warm):
+    """gd_run_corrected over T steps after `warm` plain steps, and its host
+    emulation on a second context: interval-1 steps, keep E, one more step,
+    features on the host, the same network forward, F += y / r on interior
+    cells. Returns both field sets and the last (features, network output)."""
+    with gd.Context(nc, nc, nc, dx=2.0) as ctx, gd.Net(nc, nc, nc) as net:
+        ctx.set_isotropic_eps(eps_c, dt_c)
+        ctx.add_source(gd.EZ, nc // 2, nc // 2, nc // 2, jc)
+        net.set_params(params)
+        ctx.run(warm)  # something to correct
+        ctx.run_corrected(net, T, interval, cfg)
+        got = ctx.get_fields()
+    cb = (np.float64(dt_c) / (np.asarray(eps_c, np.float64) * 2.0)).astype(np.float32)
+    ie = np.float32(1.0) / np.float32(cfg[3])
+    ih = np.float32(1.0) / np.float32(cfg[4])
+    x = y = None
+    with gd.Context(nc, nc, nc, dx=2.0) as ctx, gd.Net(nc, nc, nc) as net:
+        ctx.set_isotropic_eps(eps_c, dt_c)
+        ctx.add_source(gd.EZ, nc // 2, nc // 2, nc // 2, jc)
+        net.set_params(params)
+        ctx.run(warm)
+        done = 0
+        while done + interval <= T:
+            ctx.run(interval - 1)
+            prev = ctx.get_fields()
+            ctx.run(1)
+            cur = ctx.get_fields()
+            x = learned.features(cur, prev[:3], cb, cfg)
+            del prev
+            net.set_input(x)
+            y = net.forward()
+            for q in range(6):
+                s = ie if q < 3 else ih
+                cur[q][1:-1, 1:-1, 1:-1] = cur[q][1:-1, 1:-1, 1:-1] + y[1:-1, 1:-1, 1:-1, q] * s
+            ctx.set_fields(cur)
+            del cur
+            done += interval
+        if done < T:
+            ctx.run(T - done)
+        want = ctx.get_fields()
+    return got, want, x, y
+
+
 @pytest.mark.parametrize("interval", [2, 4])
 def test_run_corrected_matches_host_emulation(gd, interval):
     from gdiamond_amd import learned
@@ -62,42 +105,48 @@ def test_run_corrected_matches_host_emulation(gd, interval):
     cfg = (3.0, 2.5, 8.0, 40.0, 50.0, float(np.float32(2.0) / prob.dt_c))
     params = learned.init_params(5, last_scale=1.0)
     T = 3 * interval + 1
-    # device path
-    with gd.Context(nc, nc, nc, dx=2.0) as ctx, gd.Net(nc, nc, nc) as net:
-        ctx.set_isotropic_eps(prob.eps_c, prob.dt_c)
-        ctx.add_source(gd.EZ, nc // 2, nc // 2, nc // 2, prob.jc)
-        net.set_params(params)
-        ctx.run(20)  # something to correct
-        ctx.run_corrected(net, T, interval, cfg)
-        got = ctx.get_fields()
-    # host emulation
-    cb = prob.cb_coarse()
-    ie = np.float32(1.0) / np.float32(cfg[3])
-    ih = np.float32(1.0) / np.float32(cfg[4])
-    with gd.Context(nc, nc, nc, dx=2.0) as ctx, gd.Net(nc, nc, nc) as net:
-        ctx.set_isotropic_eps(prob.eps_c, prob.dt_c)
-        ctx.add_source(gd.EZ, nc // 2, nc // 2, nc // 2, prob.jc)
-        net.set_params(params)
-        ctx.run(20)
-        done = 0
-        while done + interval <= T:
-            ctx.run(interval - 1)
-            prev = ctx.get_fields()
-            ctx.run(1)
-            cur = ctx.get_fields()
-            net.set_input(learned.features(cur, prev[:3], cb, cfg))
-            y = net.forward()
-            for q in range(6):
-                s = ie if q < 3 else ih
-                cur[q][1:-1, 1:-1, 1:-1] = cur[q][1:-1, 1:-1, 1:-1] + y[1:-1, 1:-1, 1:-1, q] * s
-            ctx.set_fields(cur)
-            done += interval
-        if done < T:
-            ctx.run(T - done)
-        want = ctx.get_fields()
+    got, want, _, y = _corrected_vs_emulation(gd, learned, nc, prob.eps_c, prob.dt_c, prob.jc,
+                                              params, cfg, T, interval, 20)
     assert np.max(np.abs(y[..., :6])) > 0  # the network does correct
     for c in range(6):
         assert bits_equal(got[c], want[c]), (c, describe_diff(got[c], want[c]))
+
+
+@pytest.mark.timeout(600)
+def test_learned_form_at_its_size(gd):
+    """The learned form's own configuration: the coarse 256^3 grid (dx = 2,
+    dt = 2 dt_fine) of the 512^3 sphere medium with eps 1-6 (seed 6000, 200
+    spheres, radius 4-20 fine cells), the fine Gaussian Jz mapped to the
+    coarse grid. Two corrector intervals of 4 coarse steps plus one plain step
+    through gd_run_corrected == the host emulation bit for bit; and a 32^3
+    block of the network forward at 256^3 (the emulation's last application)
+    == the FP32 torch reference with the device's BF16 roundings, to the
+    tolerance of the network tests (2e-2 relative: BF16 activations)."""
+    import nn_ref
+    from gdiamond_amd import learned
+    nc = 256
+    eps_c = gd.sphere_medium(nc, nc, nc, 6000, 200, 2.0, 10.0, 1.0, 6.0)
+    dt_c = np.float32(2.0) * learned.DT_FINE
+    jc = learned.coarse_source(gd.gaussian_pulse_table(2 * 200 + 2, 30.0, 10.0, 0.05))
+    cfg = (3.0, 2.5, 8.0, 40.0, 50.0, float(np.float32(2.0) / dt_c))
+    params = learned.init_params(5, last_scale=1.0)
+    say("[learned 256^3] corrected run and host emulation")
+    got, want, x, y = _corrected_vs_emulation(gd, learned, nc, eps_c, dt_c, jc, params, cfg,
+                                              T=9, interval=4, warm=40)
+    assert np.max(np.abs(y[..., :6])) > 0
+    for c in range(6):
+        assert bits_equal(got[c], want[c]), (c, describe_diff(got[c], want[c]))
+    del got, want
+    # 32^3 output block around the source, with the 3-cell receptive-field halo
+    # of three 3x3x3 layers taken from the same input
+    lo, m, h = 112, 32, 3
+    xs = np.ascontiguousarray(x[lo - h:lo + m + h, lo - h:lo + m + h, lo - h:lo + m + h])
+    ref, _ = nn_ref.forward(params, xs)
+    ref = ref.detach().numpy()[h:h + m, h:h + m, h:h + m, :12]
+    blk = y[lo:lo + m, lo:lo + m, lo:lo + m, :12]
+    err = np.max(np.abs(blk - ref)) / np.max(np.abs(ref))
+    say(f"[learned 256^3] forward block rel err {err:.3e}")
+    assert np.max(np.abs(ref)) > 0 and err < 2e-2, err
 
 
 def test_zero_network_leaves_run_unchanged(gd):

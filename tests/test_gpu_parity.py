@@ -270,6 +270,35 @@ def test_dft_monitor_fp64(gpu, oracle, axis, index):
             assert bits_equal(im, wim[s]), f"variant {variant} comp {c} im: {describe_diff(im, wim[s])}"
 
 
+def test_dft_closed_form_driven_cell(gpu, oracle):
+    """A cell driven so that its Ez after step n is cos(2 pi f0 n) (Caz = 0,
+    Cbz = 1 there, every other Cb and Db zero, J(n) = -cos): the device's FP64
+    monitor sums equal the long-double geometric-series closed form to the
+    float32 rounding of the driven values, for PLAIN and TILED, over a run
+    split into three calls, and equal the oracle bit for bit."""
+    from test_oracle_invariants import _driven_cell_setup, dft_closed_form
+    shape = (21, 19, 17)
+    cell = (7, 11, 9)
+    f0, T, scale = 0.05, 397, 0.5
+    co, src = _driven_cell_setup(shape, cell, f0, T, np.float32)
+    f_cps = np.array([0.05, 0.031, 0.07, 0.0, 0.113])
+    setup = dict(coeffs=co, sources=[(gpu.EZ, *cell, src)],
+                 dft=dict(comps=[2], axis=2, index=cell[2], f_cps=f_cps, scale=scale))
+    want, (wre, wim) = oracle_run(oracle, shape, setup, T)
+    p = cell[0] + cell[1] * shape[0]
+    tol = T * scale * 2.0 ** -23
+    for variant, t in ((gpu.PLAIN, None), (gpu.TILED, dict(blt=3, rows=2, waves=4, zchunk=5)),
+                       (gpu.TILED, dict(blt=4, rows=4, waves=8, zchunk=8))):
+        say(f"[dft closed form] variant {variant} {t}")
+        got, gdft = run_variant(gpu, shape, setup, T, variant, t, chunks=[150, 100, 147])
+        assert_fields_equal(got, want, "driven cell fields")
+        re, im = gdft[2]
+        assert bits_equal(re, wre[0]) and bits_equal(im, wim[0])
+        for q, fq in enumerate(f_cps):
+            closed = dft_closed_form(f0, fq, 0, T, scale)
+            assert abs(complex(re[q, p], im[q, p]) - closed) <= tol, (variant, fq)
+
+
 # ------------------------------------------------------- benchmark workloads, full size
 
 
@@ -293,30 +322,40 @@ def test_config_vacuum_128_200_steps(gpu, oracle):
         assert_fields_equal(got, want, f"c1 variant {variant}")
 
 
-def test_config_spheres_512_tiled_vs_plain_and_oracle(gpu, oracle):
-    """Full-size 512^3 sphere medium (200 spheres, r 4-20, eps 1-12, seed 42):
-    6 steps vs the oracle (OpenMP) bit for bit, then 60 steps TILED vs PLAIN bit
-    for bit including the FP64 DFT on the z=256 plane."""
+@pytest.mark.timeout(900)
+def test_config_spheres_512_full_run_vs_oracle(gpu, oracle):
+    """The 512^3 sphere workload (200 spheres, r 4-20, eps 1-12, seed 42,
+    Gaussian Jz at the centre, FP64 DFT of Ex and Ez on z = 256 at
+    0.03/0.05/0.07 cycles/step) for all 1000 steps: TILED (the planner's
+    tiling, the benchmark's material form) and PLAIN each equal the oracle's
+    OpenMP update of the whole run on every element of the six fields and of
+    the DFT sums, bit for bit -- the reference's whole-run relation
+    (unittests/checker_gpu.cu:9-22, gdiamond.hpp:1330-1347)."""
     shape = (512, 512, 512)
+    T = 1000
     eps, cb, db = iso_problem(gpu, shape, seed=42, nspheres=200, rmin=4, rmax=20, emin=1,
                               emax=12)
     del eps
-    T0 = 6
-    src = gpu.gaussian_pulse_table(1000, 30.0, 10.0, 0.05)
-    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)])
-    say("[512] oracle 6 steps")
-    want, _ = oracle_run(oracle, shape, setup, T0, threads=16)
-    say("[512] tiled 6 steps")
-    got, _ = run_variant(gpu, shape, setup, T0, gpu.TILED)
-    assert_fields_equal(got, want, "512 tiled vs oracle")
-    del want, got
-    setup["dft"] = dict(comps=[0, 2], axis=2, index=256, f_cps=np.array([0.03, 0.05, 0.07]))
-    T1 = 60
-    say("[512] plain 60 steps")
-    a, adft = run_variant(gpu, shape, setup, T1, gpu.PLAIN)
-    say("[512] tiled 60 steps")
-    b, bdft = run_variant(gpu, shape, setup, T1, gpu.TILED)
-    assert_fields_equal(b, a, "512 tiled vs plain")
-    for c in (0, 2):
-        assert bits_equal(bdft[c][0], adft[c][0]), describe_diff(bdft[c][0], adft[c][0])
-        assert bits_equal(bdft[c][1], adft[c][1]), describe_diff(bdft[c][1], adft[c][1])
+    src = gpu.gaussian_pulse_table(T, 30.0, 10.0, 0.05)
+    f_cps = np.array([0.03, 0.05, 0.07])
+    setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)],
+                 dft=dict(comps=[0, 2], axis=2, index=256, f_cps=f_cps))
+    got = {}
+    for variant in (gpu.TILED, gpu.PLAIN):
+        say(f"[512 x {T}] variant {variant}")
+        got[variant] = run_variant(gpu, shape, setup, T, variant)
+    # the oracle in chunks of 100 steps (16 threads), continuing its DFT sums
+    want = oracle.zero_fields(shape)
+    d = dict(setup["dft"])
+    for n0 in range(0, T, 100):
+        res = oracle.run(shape, want, cb=cb, db=db, dx=1.0, sources=setup["sources"], n0=n0,
+                         nsteps=100, dft=d, threads=16)
+        d["re"], d["im"] = res
+        say(f"[512 x {T}] oracle at step {n0 + 100}")
+    for variant, (fields, dft) in got.items():
+        assert_fields_equal(fields, want, f"512 x {T} variant {variant} vs oracle")
+        for s, c in enumerate((0, 2)):
+            re, im = dft[c]
+            assert bits_equal(re, d["re"][s]), (variant, c, describe_diff(re, d["re"][s]))
+            assert bits_equal(im, d["im"][s]), (variant, c, describe_diff(im, d["im"][s]))
+    assert np.abs(d["re"]).max() > 0

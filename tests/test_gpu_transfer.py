@@ -49,7 +49,12 @@ def test_other_storage_reads_match_fp64_read(gd, shape, storage):
         got = ctx.get_fields()
         for a in range(6):
             assert bits_equal(got[a], want[a]), a
-        if storage != "f64":
+        if storage == "f64":
+            # FP64 holds every float32 input exactly: the upload and both
+            # read-backs must return the input itself
+            for a in range(6):
+                assert bits_equal(got[a], f[a]), a
+        else:
             # the stored values are a rounding of the input, not the input
             assert not bits_equal(got[0], f[0])
 

@@ -107,8 +107,9 @@ def test_oracle_plants_detected(oracle):
 
 def test_reference_build_reports_its_seq_rate(oracle):
     """The reference's own update_FDTD_seq, built from its sources into
-    oracle/_ref, runs and reports the Mcells/s it prints (the bench's
-    reference CPU sample); skipped where that build is absent."""
+    oracle/_ref, runs and reports the Mcells/s it prints (its rate is quoted
+    as context beside the CPU baseline); skipped where that build is
+    absent."""
     import ctypes as C
     import os
     if not os.path.exists(oracle.REF_LIB_PATH):

@@ -8,7 +8,10 @@
   div(eps E) are constant in time away from sources and walls.
 * DFT convention (kernels.cuh:87-113): re += v cos(2 pi f n) s, im -= v sin(..) s
   with v the value after step n, checked against a direct numpy sum over the
-  recorded time series.
+  recorded time series, and against the closed form: a cell driven so that its
+  value after step n is cos(2 pi f0 n) accumulates s times a sum of two
+  geometric series, evaluated in long double (a wrong sign, a step index off
+  by one or a dropped scale fails it).
 * FP32 and FP64 oracles agree to single-precision rounding.
 """
 import numpy as np
@@ -179,3 +182,91 @@ def test_dft_weights_are_separate_libm_cos_sin(oracle):
             wc, ws = oracle.dft_weights(f, n, 1.0)
             bad += (wc != math.cos(ang)) + (ws != math.sin(ang))
     assert bad == 0
+
+
+# ------------------------------------------------------------ DFT closed form
+
+
+def dft_closed_form(f0, f, n0, n1, scale):
+    """s * sum_{n=n0}^{n1-1} cos(2 pi f0 n) exp(-2 pi i f n) in long double:
+    cos = (e^{i a n} + e^{-i a n}) / 2, each half a geometric series
+    sum_{n=n0}^{n1-1} q^n = q^n0 (1 - q^(n1-n0)) / (1 - q)."""
+    two_pi = np.longdouble(2) * np.longdouble(np.pi)
+    a = two_pi * np.longdouble(f0)
+    w = two_pi * np.longdouble(f)
+    tot = np.clongdouble(0)
+    for phi in (a - w, -a - w):
+        q = np.exp(np.clongdouble(1j) * phi)
+        if abs(1 - q) < 1e-15:
+            g = np.clongdouble(n1 - n0)
+        else:
+            g = q ** n0 * (1 - q ** (n1 - n0)) / (1 - q)
+        tot += g / 2
+    return complex(tot * np.longdouble(scale))
+
+
+def _driven_cell_setup(shape, cell, f0, T, dtype):
+    """Full (12-array) material that makes the value of Ez at `cell` after
+    step n equal -J(n) * dx with J(n) = -cos(2 pi f0 n): Caz = 0, Cbz = 1 at
+    the cell, Cb = 0 elsewhere and Db = 0 everywhere (H stays 0, so the curl
+    is 0 and E = 0 * E + 1 * (0 - J dx)); every other E keeps 0."""
+    nx, ny, nz = shape
+    i, j, k = cell
+    co = [np.zeros((nz, ny, nx), np.float32) for _ in range(12)]
+    for a in (0, 2, 4, 6, 8, 10):
+        co[a][:] = 1.0  # Ca, Da
+    co[4][k, j, i] = 0.0
+    co[5][k, j, i] = 1.0
+    n = np.arange(T)
+    src = (-np.cos(2 * np.pi * f0 * n)).astype(np.float32)
+    return co, src
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dft_closed_form(oracle, dtype):
+    shape = (9, 8, 7)
+    cell = (4, 3, 3)
+    f0 = 0.05
+    T = 397  # not a whole number of periods: every frequency has a sizeable sum
+    scale = 0.5
+    co, src = _driven_cell_setup(shape, cell, f0, T, dtype)
+    f_cps = np.array([0.05, 0.031, 0.07, 0.0, 0.113])
+    f = oracle.zero_fields(shape, dtype)
+    # two calls: the weights follow the absolute step n0 + r
+    dft = dict(comps=[2], axis=2, index=cell[2], f_cps=f_cps, scale=scale)
+    re, im = oracle.run(shape, f, coeffs=co, dx=1.0, sources=[(2, *cell, src)], nsteps=150,
+                        dft=dft, dtype=dtype)
+    dft.update(re=re, im=im)
+    oracle.run(shape, f, coeffs=co, dx=1.0, sources=[(2, *cell, src)], n0=150, nsteps=T - 150,
+               dft=dft, dtype=dtype)
+    p = cell[0] + cell[1] * shape[0]
+    # the driven value is cos rounded to float32 (source tables are float32 in
+    # both oracles: <= 2^-24 relative); the FP64 sums add T such terms
+    tol = T * scale * 2.0 ** -23
+    for q, fq in enumerate(f_cps):
+        want = dft_closed_form(f0, fq, 0, T, scale)
+        got = complex(re[0, q, p], im[0, q, p])
+        assert abs(got - want) <= tol, (fq, got, want)
+        # the checks are discriminating: one step late, the opposite sign of
+        # the imaginary part or no scale each miss by far more than tol
+        late = dft_closed_form(f0, fq, 1, T + 1, scale) * np.exp(2j * np.pi * fq)
+        assert abs(got - late) > 100 * tol or abs(want) < 1.0
+        assert abs(got - want.conjugate()) > 100 * tol or abs(want.imag) < 1.0
+        assert abs(got - want / scale) > 100 * tol or abs(want) < 1.0
+    # every other cell of the plane stays 0
+    mask = np.ones(re.shape[-1], bool)
+    mask[p] = False
+    assert not re[0][:, mask].any() and not im[0][:, mask].any()
+
+
+def test_dft_closed_form_resonance_magnitude():
+    """Sanity of the closed form itself: at f = f0 the sum grows as T/2, away
+    from resonance it stays bounded; brute force in long double agrees."""
+    for f0, f, T in ((0.05, 0.05, 400), (0.05, 0.031, 400), (0.07, 0.02, 123)):
+        n = np.arange(T, dtype=np.longdouble)
+        two_pi = np.longdouble(2) * np.longdouble(np.pi)
+        e = np.cos(two_pi * np.longdouble(f0) * n)
+        brute = complex(np.sum(e * np.cos(two_pi * np.longdouble(f) * n)),
+                        -np.sum(e * np.sin(two_pi * np.longdouble(f) * n)))
+        assert abs(dft_closed_form(f0, f, 0, T, 1.0) - brute) < 1e-9
+    assert abs(dft_closed_form(0.05, 0.05, 0, 400, 1.0) - 200.0) < 1.0
