@@ -50,6 +50,31 @@ _FP = C.POINTER(C.c_float)
 _DP = C.POINTER(C.c_double)
 _lib = None
 
+
+class PointSource(C.Structure):
+    """gd_point_source: a point source whose value table is a device array."""
+    _fields_ = [("comp", C.c_int), ("i", C.c_int), ("j", C.c_int), ("k", C.c_int),
+                ("table", C.c_void_p)]
+
+
+class NoallocLaunch(C.Structure):
+    """gd_noalloc_launch: one TILED launch on caller-owned device buffers
+    (gd_tiled_launch_noalloc). Pointer fields other than `sources` and
+    `f_cycles_per_step` are device addresses."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("fields_in", C.c_void_p * 6), ("fields_out", C.c_void_p * 6),
+                ("cb", C.c_void_p), ("material_id", C.c_void_p),
+                ("material_table", C.c_void_p), ("db", C.c_float), ("nsrc", C.c_int),
+                ("sources", C.POINTER(PointSource)), ("n0", C.c_int), ("nsteps", C.c_int),
+                ("dft_mask", C.c_uint), ("dft_axis", C.c_int), ("dft_index", C.c_int),
+                ("nf", C.c_int), ("f_cycles_per_step", _DP), ("dft_scale", C.c_double),
+                ("dft_re", C.c_void_p), ("dft_im", C.c_void_p), ("stream", C.c_void_p)]
+
+
+NOALLOC_MAX_STEPS = 4
+NOALLOC_MAX_FREQS = 16
+NOALLOC_MAX_SNAP_CELLS = 1 << 21
+
 # name -> (restype, argtypes)
 _SIGS = {
     "gd_version": (C.c_char_p, []),
@@ -134,6 +159,7 @@ _SIGS = {
     "gd_sphere_medium": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, C.c_int,
                                    C.c_double, C.c_double, C.c_double, C.c_double, _FP]),
     "gd_gaussian_pulse_table": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, _FP]),
+    "gd_tiled_launch_noalloc": (C.c_int, [C.POINTER(NoallocLaunch)]),
 }
 
 

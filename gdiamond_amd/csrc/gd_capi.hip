@@ -394,11 +394,18 @@ gd_tiling plan(const gd_ctx* c) { return plan_dims(c->nx, c->ny, c->nz, c->stora
 
 namespace gd {
 // DFT weights of absolute step n: cos(2 pi f n)*scale and sin(2 pi f n)*scale
-// in double, as every run evaluates them.
+// in double, as every run evaluates them (shared with the allocation-free
+// entry point so that both give the same sums).
 void dft_weight(double f_cps, long long n, double scale, double* wc, double* ws) {
   const double ang = 2.0 * 3.14159265358979323846 * f_cps * (double)n;
   *wc = weight_cos(ang) * scale;
   *ws = weight_sin(ang) * scale;
+}
+// Error text of calls made without a context (gd_last_error(NULL)).
+void set_call_error(const char* msg) { g_create_error = msg; }
+// The planner's tiling (shared with the allocation-free entry point).
+gd_tiling plan_tiling_dims(size_t nx, size_t ny, size_t nz, int storage) {
+  return plan_dims(nx, ny, nz, storage);
 }
 }  // namespace gd
 

@@ -223,6 +223,60 @@ int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
  * whose tile sizes are compile-time constants (kernels_mm_ver2.cuh:8-32). */
 int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* out);
 
+/* ---- one TILED launch that allocates nothing ---------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one TILED launch of nsteps (1..GD_NOALLOC_MAX_STEPS)
+ * steps on caller-owned device buffers, with FP32 fields and the isotropic
+ * material (dense Cb, or 1-byte ids and a table of GD_MAX_MATERIALS floats).
+ * It enqueues on `stream` and returns without waiting; it calls no hipMalloc,
+ * hipFree or other allocating routine. Its scratch memory (source
+ * descriptors, DFT weights, the per-level snapshot of the monitored plane) is
+ * one fixed device array of 4 KiB + 4 * GD_NOALLOC_MAX_SNAP_CELLS floats
+ * (32 MiB) that each GPU keeps until the process ends; every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams or
+ * threads) race. The first call on a device looks up the array's address:
+ * make it before capturing a graph. Sizes beyond the limits are rejected with
+ * GD_ERR_ARG (text in gd_last_error(NULL)). fields_out receives every cell
+ * (boundary cells copied) and must not overlap fields_in; the DFT sums are
+ * accumulated in place with the weights of absolute steps n0..n0+nsteps-1,
+ * exactly as gd_run does, so a sequence of launches with n0 advancing by
+ * nsteps and the field buffers swapped equals gd_run bit for bit. */
+#define GD_NOALLOC_MAX_STEPS 4
+#define GD_NOALLOC_MAX_FREQS 16
+#define GD_NOALLOC_MAX_SNAP_CELLS (1 << 21) /* monitored components x plane cells */
+
+typedef struct gd_point_source {
+  int comp;           /* component 0..5 */
+  int i, j, k;        /* cell */
+  const float* table; /* device array: the value entering step n is table[n] */
+} gd_point_source;
+
+typedef struct gd_noalloc_launch {
+  int nx, ny, nz;
+  float dx;
+  const float* fields_in[6]; /* device: Ex Ey Ez Hx Hy Hz before step n0 */
+  float* fields_out[6];      /* device: the fields after step n0 + nsteps - 1 */
+  const float* cb;           /* device: Cb per cell, used when material_id is NULL */
+  const uint8_t* material_id;   /* device: material id per cell, or NULL */
+  const float* material_table;  /* device: Cb of each id, GD_MAX_MATERIALS floats */
+  float db;                  /* Db (H coefficient, scalar) */
+  int nsrc;                  /* 0..GD_MAX_SOURCES point sources */
+  const gd_point_source* sources; /* host array of nsrc descriptors */
+  int n0;                    /* absolute step of the first of the nsteps steps */
+  int nsteps;                /* 1..GD_NOALLOC_MAX_STEPS */
+  unsigned dft_mask;         /* monitored components (bit c); 0: no monitor */
+  int dft_axis, dft_index;   /* plane, as gd_dft_config */
+  int nf;                    /* 1..GD_NOALLOC_MAX_FREQS */
+  const double* f_cycles_per_step; /* host: nf frequencies */
+  double dft_scale;
+  double* dft_re;            /* device: [popcount(mask)][nf][plane] FP64 sums */
+  double* dft_im;
+  void* stream;              /* hipStream_t; NULL is the null stream */
+} gd_noalloc_launch;
+
+int gd_tiled_launch_noalloc(const gd_noalloc_launch* launch);
+
 /* Timing of the last gd_run, from HIP events on the context's stream:
  * total_ms spans the whole stepping loop; with per-launch timing enabled
  * (gd_set_launch_timing) kernel_ms is the sum of the durations of the

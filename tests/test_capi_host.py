@@ -160,3 +160,58 @@ def test_library_has_no_oracle_dependency(gd):
     src = "".join(open(os.path.join(ROOT, "gdiamond_amd", "csrc", f)).read()
                   for f in os.listdir(os.path.join(ROOT, "gdiamond_amd", "csrc")))
     assert "oracle" not in src.lower().replace("the oracle", "")
+
+
+def test_noalloc_launch_struct_layout(gd, tmp_path):
+    """The ctypes mirror of gd_noalloc_launch / gd_point_source has the C
+    struct's size and field offsets (compiled from the header with gcc)."""
+    import subprocess
+    fields = [f[0] for f in gd.NoallocLaunch._fields_]
+    src = ["#include <stddef.h>", "#include <stdio.h>", '#include "gdiamond/gd_capi.h"',
+           "int main(void) {",
+           '  printf("%zu %zu\\n", sizeof(gd_noalloc_launch), sizeof(gd_point_source));']
+    for f in fields:
+        src.append(f'  printf("%zu\\n", offsetof(gd_noalloc_launch, {f}));')
+    src += ["  return 0;", "}"]
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert int(out[0]) == C.sizeof(gd.NoallocLaunch)
+    assert int(out[1]) == C.sizeof(gd.PointSource)
+    for f, off in zip(fields, out[2:]):
+        assert int(off) == getattr(gd.NoallocLaunch, f).offset, f
+
+
+def test_noalloc_launch_rejects_bad_sizes_on_the_host(gd):
+    """Argument checks run before any device work: out-of-range sizes give
+    GD_ERR_ARG with a message (no GPU needed, nothing is launched)."""
+    def launch(**kw):
+        L = gd.NoallocLaunch()
+        L.nx, L.ny, L.nz, L.dx, L.nsteps = 16, 16, 16, 1.0, 4
+        for a in range(6):
+            L.fields_in[a] = 0x1000 + 0x100 * a
+            L.fields_out[a] = 0x2000 + 0x100 * a
+        L.cb = 0x3000
+        for k, v in kw.items():
+            setattr(L, k, v)
+        return gd.lib().gd_tiled_launch_noalloc(C.byref(L))
+    assert launch(nsteps=5) == -1
+    assert b"nsteps" in gd.lib().gd_last_error(None)
+    assert launch(nsteps=0) == -1
+    assert launch(nx=2) == -1
+    assert launch(nsrc=9) == -1
+    assert launch(cb=None) == -1
+    assert launch(dft_mask=3, dft_axis=2, dft_index=4, nf=17) == -1
+    assert launch(nx=2048, ny=2048, nz=3, dft_mask=3, dft_axis=2, dft_index=1, nf=1) == -1
+    L = gd.NoallocLaunch()
+    L.nx = L.ny = L.nz = 16
+    L.nsteps = 1
+    for a in range(6):
+        L.fields_in[a] = 0x1000 + 0x100 * a
+        L.fields_out[a] = 0x1000 + 0x100 * a  # in place
+    L.cb = 0x3000
+    assert gd.lib().gd_tiled_launch_noalloc(C.byref(L)) == -1
+    assert b"must not be" in gd.lib().gd_last_error(None)
