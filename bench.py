@@ -51,7 +51,7 @@ PLAIN_KERNELS = "plain_e_kernel + plain_h_kernel (one pair per step)"
 METRIC = "Gcell-updates/s at 512\u00b3 + % HBM roofline; chosen vs plain speedup at fixed error"
 
 
-def parse_traffic(path, want):
+def parse_traffic(path, want, key="bytes_per_launch"):
     """HBM bytes per launch of a kernel from a committed rocprofv3 PMC summary
     (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration). Used only
     when the summary holds every line of `want` (the tiling, size, storage and
@@ -63,7 +63,7 @@ def parse_traffic(path, want):
         for ln in f:
             if ln.strip() in want:
                 seen.add(ln.strip())
-            if ln.startswith("bytes_per_launch:"):
+            if ln.startswith(key + ":"):
                 val = float(ln.split(":", 1)[1])
     return val if seen == set(want) else None
 
@@ -426,10 +426,13 @@ def run_batch(gd, batch_size=256, T=500, steps=2, warmup=1,
     # the persistent launches of one run: per launch group, one launch per
     # blt steps (a full-kernel and a lean-kernel dispatch), plus the advance
     nlaunch = g["launches_per_run"] - 1
+    # measured bytes of a whole run (the autotuner's picks, and with them the
+    # number of launches, can differ between runs); per launch for this run
     traffic_name = f"zstream_batch_{batch_size}inst_traffic.txt"
-    traffic = parse_traffic(os.path.join(ROOT, "profiles", traffic_name),
-                            {f"instances: {batch_size}", f"timesteps: {T}",
-                             f"material: {material}"})
+    run_bytes = parse_traffic(os.path.join(ROOT, "profiles", traffic_name),
+                              {f"instances: {batch_size}", f"timesteps: {T}",
+                               f"material: {material}"}, key="bytes_per_run")
+    traffic = run_bytes / nlaunch if run_bytes else None
     return {
         "metric": "Gcell-updates/s, batched independent problems",
         "value": g["GCUPS"],
