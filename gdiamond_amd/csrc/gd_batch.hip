@@ -48,22 +48,22 @@ struct SizeClass {
 };
 
 // One launch configuration (a tiling) over a set of instances: per-instance
-// launch arguments for both copy parities and the work items. Items whose
-// tile holds a source cell come first (nextra of them, run by the full batch
-// kernel); the others run the lean build without source code.
+// launch arguments for both copy parities and the work items, instance by
+// instance (largest first), each instance's tiles in tile order. Items whose
+// tile holds a source cell are flagged: the batch kernel runs them with the
+// point-source code, all others with the build without it.
 struct Plan {
   gd::ZTiling t{};
   gd::ZArgs* d_args = nullptr;   // [2 * ninst]
   gd::BatchItem* d_items = nullptr;
   int nitems = 0;
-  int nextra = 0;
-  int nblocks = 0;       // full kernel
-  int nblocks_lean = 0;  // lean kernel
+  int nfull = 0;    // flagged items
+  int nblocks = 0;  // workgroups of a launch
   std::vector<int> insts;
 };
 
-// Queue counters of one launch: one for the full kernel, one for the lean one.
-constexpr int kCountersPerLaunch = 2;
+// Queue counters of one launch.
+constexpr int kCountersPerLaunch = 1;
 
 struct Group {
   int blt = 0, rows = 0, waves = 0;
@@ -227,7 +227,7 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
   const size_t ninst = b->inst.size();
   std::vector<gd::ZArgs> args(2 * ninst);
   std::memset(args.data(), 0, args.size() * sizeof(gd::ZArgs));
-  std::vector<gd::BatchItem> items, lean_items;
+  std::vector<gd::BatchItem> items;
   std::vector<int> extra_ids, lean_ids;
   std::vector<gd::SourceDesc> sd;
   gd::Dft no_dft{};
@@ -239,17 +239,22 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
     const gd::ZArgs& a = args[2 * i];
     const long long nt = (long long)a.ntx * a.nty * a.ntz;
     if (nt > (1ll << 30)) return b->fail(GD_ERR_ARG, "too many tiles");
-    // tiles with a source cell in their window need the full kernel (the
-    // batch tile order is x fastest, then y, then z: ids are positions)
+    // tiles with a source cell in their window run with the source code (the
+    // batch tile order is x fastest, then y, then z: ids are positions, and
+    // both lists come in that order)
     sd.clear();
     for (const auto& s : b->inst[i].src)
       sd.push_back(gd::SourceDesc{s.comp, (int)s.i, (int)s.j, (int)s.k, s.table});
     gd::zstream_classify(ti, 0, a.g, sd.data(), (int)sd.size(), no_dft, extra_ids, lean_ids);
-    for (int q : extra_ids) items.push_back(gd::BatchItem{i, q});
-    for (int q : lean_ids) lean_items.push_back(gd::BatchItem{i, q});
+    size_t qe = 0, ql = 0;
+    while (qe < extra_ids.size() || ql < lean_ids.size()) {
+      const bool e = ql == lean_ids.size() ||
+                     (qe < extra_ids.size() && extra_ids[qe] < lean_ids[ql]);
+      items.push_back(e ? gd::BatchItem{i, extra_ids[qe++], 1}
+                        : gd::BatchItem{i, lean_ids[ql++], 0});
+    }
+    p.nfull += (int)extra_ids.size();
   }
-  p.nextra = (int)items.size();
-  items.insert(items.end(), lean_items.begin(), lean_items.end());
   if (items.size() > (size_t)INT32_MAX) return b->fail(GD_ERR_ARG, "too many work items");
   BH(b, hipMalloc(&p.d_args, args.size() * sizeof(gd::ZArgs)));
   BH(b, hipMalloc(&p.d_items, std::max<size_t>(1, items.size()) * sizeof(gd::BatchItem)));
@@ -258,9 +263,8 @@ int build_plan(gd_batch* b, const std::vector<int>& insts, const gd::ZTiling& t,
     BH(b, hipMemcpy(p.d_items, items.data(), items.size() * sizeof(gd::BatchItem),
                     hipMemcpyHostToDevice));
   p.nitems = (int)items.size();
-  p.nblocks = gd::zstream_batch_blocks(t, batch_mat(b), false, b->device);
-  p.nblocks_lean = gd::zstream_batch_blocks(t, batch_mat(b), true, b->device);
-  if (p.nblocks < 1 || p.nblocks_lean < 1) return b->fail(GD_ERR_HIP, "occupancy query failed");
+  p.nblocks = gd::zstream_batch_blocks(t, batch_mat(b), b->device);
+  if (p.nblocks < 1) return b->fail(GD_ERR_HIP, "occupancy query failed");
   return GD_OK;
 }
 
@@ -385,28 +389,18 @@ int ensure_streams(gd_batch* b, size_t n) {
   return GD_OK;
 }
 
-// One launch of plan p: the full kernel over the items with source tiles,
-// then the lean kernel over the others, on `stream`, each with its queue
-// counter (counters[0], counters[1]). L carries the other launch fields; nk
-// counts the kernels issued.
-int launch_plan(gd_batch* b, const Plan& p, gd::BatchLaunch L, unsigned* counters,
+// One launch of plan p on `stream`: one persistent kernel over every item,
+// with its queue counter. L carries the other launch fields; nk counts the
+// kernels issued.
+int launch_plan(gd_batch* b, const Plan& p, gd::BatchLaunch L, unsigned* counter,
                 hipStream_t stream, size_t* nk) {
-  const gd::MatKind mk = batch_mat(b);
+  if (p.nitems < 1) return GD_OK;
   L.inst = p.d_args;
-  if (p.nextra > 0) {
-    L.items = p.d_items;
-    L.nitems = p.nextra;
-    L.counter = counters;
-    BH(b, gd::launch_zstream_batch(p.t, mk, false, L, p.nblocks, stream));
-    ++*nk;
-  }
-  if (p.nitems > p.nextra) {
-    L.items = p.d_items + p.nextra;
-    L.nitems = p.nitems - p.nextra;
-    L.counter = counters + 1;
-    BH(b, gd::launch_zstream_batch(p.t, mk, true, L, p.nblocks_lean, stream));
-    ++*nk;
-  }
+  L.items = p.d_items;
+  L.nitems = p.nitems;
+  L.counter = counter;
+  BH(b, gd::launch_zstream_batch(p.t, batch_mat(b), L, p.nblocks, stream));
+  ++*nk;
   return GD_OK;
 }
 

@@ -382,6 +382,7 @@ inline void ztile_counts(const ZTiling& t, int nx, int ny, int nz, int& ntx, int
 struct BatchItem {
   int inst;
   int tile;
+  int full;  // 1: the tile holds a source cell (runs the build with source code)
 };
 struct BatchLaunch {
   const ZArgs* inst;     // [2 * ninst]: src = copy p, dst = copy 1-p for parity p
@@ -393,11 +394,12 @@ struct BatchLaunch {
   int step_offset;       // steps of the run before this launch
   int launch_index;      // launches of the group before this one in the run
 };
-// lean: the items' tiles hold no source cell (the build without source code).
-hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, bool lean, const BatchLaunch& L,
-                                int nblocks, hipStream_t stream);
+// One persistent launch over the items of L; each item runs the tile with or
+// without the point-source code as its `full` flag says.
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
+                                hipStream_t stream);
 // Workgroups of one batch launch that fill the device (occupancy x CUs).
-int zstream_batch_blocks(const ZTiling& t, MatKind mk, bool lean, int device);
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device);
 // End of a run: step += nsteps, parity[g] ^= flip[g], every queue counter = 0.
 hipError_t launch_batch_advance(int* step, int nsteps, int* parity, const int* flip, int ngroups,
                                 unsigned* counters, int ncounters, hipStream_t stream);

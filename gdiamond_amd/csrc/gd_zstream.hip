@@ -144,6 +144,26 @@ __host__ __device__ __forceinline__ void tile_coords(const ZArgs& a, const int t
   }
 }
 
+// LDS of one tile: exchange rows of every level and the Cb table of the
+// indexed material. One object per (levels, waves, material form), shared by
+// both builds of the tile, so that a kernel holding both (the batch kernel)
+// allocates it once.
+template <int B, int W, bool kIdx>
+struct ZLds {
+  // h[level-1][parity of p][wave][Hz,Hx][lane]: last row of H of a level,
+  // read by the next wave for the j-1 neighbour of the next level.
+  // e[level-1][parity of plane][wave][Ez,Ex][lane]: first row of E of a
+  // level, read by the previous wave one stream position later (j+1).
+  float h[B][2][W][2][64];
+  float e[B][2][W][2][64];
+  float cb[kIdx ? kMaxMaterials : 1];  // Cb of each material id
+};
+template <int B, int W, bool kIdx>
+__device__ __forceinline__ ZLds<B, W, kIdx>& zlds() {
+  __shared__ ZLds<B, W, kIdx> lds;
+  return lds;
+}
+
 // One workgroup's work: tile (tx, ty, tz) of the problem described by `a`,
 // streamed through B time levels. kX = false compiles out the point sources
 // and the DFT snapshots (tiles without a source cell in their window and
@@ -159,14 +179,10 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   constexpr unsigned kLogES = ES == 4 ? 2 : (ES == 2 ? 1 : 0);
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
-  // lds_h[level-1][parity of p][wave][Hz,Hx][lane]: last row of H of a level,
-  // read by the next wave for the j-1 neighbour of the next level.
-  // lds_e[level-1][parity of plane][wave][Ez,Ex][lane]: first row of E of a
-  // level, read by the previous wave one stream position later (j+1).
-  __shared__ float lds_h[B][2][W][2][64];
-  __shared__ float lds_e[B][2][W][2][64];
-  // Cb of each material id (indexed material only)
-  __shared__ float lds_cb[kIdx ? kMaxMaterials : 1];
+  ZLds<B, W, kIdx>& lds = zlds<B, W, kIdx>();
+  auto& lds_h = lds.h;
+  auto& lds_e = lds.e;
+  auto& lds_cb = lds.cb;
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -551,8 +567,10 @@ __device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
 // launch's queue until it runs dry (every wave of a workgroup sees the same
 // item, so all of them leave the loop together). Items of the largest
 // instances come first, so the tail of the launch is made of small tiles.
-// kX = false: the items' tiles hold no source cell (the lean build).
-template <int B, int R, int W, int kMat, class S, bool kX>
+// Items whose tile holds a source cell run the tile with the point-source code,
+// all others the build without it: both loops are in the kernel, the item's
+// flag (uniform) picks one, and each keeps its own register state.
+template <int B, int R, int W, int kMat, class S>
 __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_batch_kernel(
     BatchLaunch L) {
   __shared__ int s_item;
@@ -574,7 +592,10 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_b
     a.n0 = n0;
     int tx, ty, tz;
     tile_coords(a, wi.tile, tx, ty, tz);
-    zstream_tile<B, R, W, kMat, S, kX>(a, tx, ty, tz);
+    if (wi.full)
+      zstream_tile<B, R, W, kMat, S, true>(a, tx, ty, tz);
+    else
+      zstream_tile<B, R, W, kMat, S, false>(a, tx, ty, tz);
   }
 }
 
@@ -587,64 +608,62 @@ __global__ __launch_bounds__(256) void batch_advance_kernel(int* step, int nstep
   for (int c = t; c < ncounters; c += blockDim.x) counters[c] = 0u;
 }
 
-template <int B, int R, int W, int kMat, bool kX>
+template <int B, int R, int W, int kMat>
 void launch_batch_k(const BatchLaunch& L, int nblocks, hipStream_t stream) {
-  hipLaunchKernelGGL((zstream_batch_kernel<B, R, W, kMat, float, kX>), dim3(nblocks),
-                     dim3(64 * W), 0, stream, L);
+  hipLaunchKernelGGL((zstream_batch_kernel<B, R, W, kMat, float>), dim3(nblocks), dim3(64 * W), 0,
+                     stream, L);
 }
 
-template <int B, int R, int W, bool kX>
+template <int B, int R, int W>
 const void* batch_kernel_fn(MatKind mk) {
   return mk == kMatIdx
-             ? reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIdx, float, kX>)
-             : reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIso, float, kX>);
+             ? reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIdx, float>)
+             : reinterpret_cast<const void*>(&zstream_batch_kernel<B, R, W, kMatIso, float>);
 }
 
-template <int R, int W, bool kX>
+template <int R, int W>
 const void* batch_kernel_rw(int blt, MatKind mk) {
   switch (blt) {
-    case 1: return batch_kernel_fn<1, R, W, kX>(mk);
-    case 2: return batch_kernel_fn<2, R, W, kX>(mk);
-    case 3: return batch_kernel_fn<3, R, W, kX>(mk);
-    case 4: return batch_kernel_fn<4, R, W, kX>(mk);
+    case 1: return batch_kernel_fn<1, R, W>(mk);
+    case 2: return batch_kernel_fn<2, R, W>(mk);
+    case 3: return batch_kernel_fn<3, R, W>(mk);
+    case 4: return batch_kernel_fn<4, R, W>(mk);
     default: return nullptr;
   }
 }
 
-template <bool kX>
 const void* batch_kernel(const ZTiling& t, MatKind mk) {
-  if (t.rows == 2 && t.waves == 4) return batch_kernel_rw<2, 4, kX>(t.blt, mk);
-  if (t.rows == 2 && t.waves == 8) return batch_kernel_rw<2, 8, kX>(t.blt, mk);
-  if (t.rows == 4 && t.waves == 4) return batch_kernel_rw<4, 4, kX>(t.blt, mk);
-  if (t.rows == 4 && t.waves == 8) return batch_kernel_rw<4, 8, kX>(t.blt, mk);
+  if (t.rows == 2 && t.waves == 4) return batch_kernel_rw<2, 4>(t.blt, mk);
+  if (t.rows == 2 && t.waves == 8) return batch_kernel_rw<2, 8>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 4) return batch_kernel_rw<4, 4>(t.blt, mk);
+  if (t.rows == 4 && t.waves == 8) return batch_kernel_rw<4, 8>(t.blt, mk);
   return nullptr;
 }
 
-template <int R, int W, bool kX>
+template <int R, int W>
 hipError_t launch_batch_rw(int blt, MatKind mk, const BatchLaunch& L, int nblocks,
                            hipStream_t stream) {
   const bool idx = mk == kMatIdx;
   switch (blt) {
-    case 1: idx ? launch_batch_k<1, R, W, kMatIdx, kX>(L, nblocks, stream)
-                : launch_batch_k<1, R, W, kMatIso, kX>(L, nblocks, stream); break;
-    case 2: idx ? launch_batch_k<2, R, W, kMatIdx, kX>(L, nblocks, stream)
-                : launch_batch_k<2, R, W, kMatIso, kX>(L, nblocks, stream); break;
-    case 3: idx ? launch_batch_k<3, R, W, kMatIdx, kX>(L, nblocks, stream)
-                : launch_batch_k<3, R, W, kMatIso, kX>(L, nblocks, stream); break;
-    case 4: idx ? launch_batch_k<4, R, W, kMatIdx, kX>(L, nblocks, stream)
-                : launch_batch_k<4, R, W, kMatIso, kX>(L, nblocks, stream); break;
+    case 1: idx ? launch_batch_k<1, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<1, R, W, kMatIso>(L, nblocks, stream); break;
+    case 2: idx ? launch_batch_k<2, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<2, R, W, kMatIso>(L, nblocks, stream); break;
+    case 3: idx ? launch_batch_k<3, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<3, R, W, kMatIso>(L, nblocks, stream); break;
+    case 4: idx ? launch_batch_k<4, R, W, kMatIdx>(L, nblocks, stream)
+                : launch_batch_k<4, R, W, kMatIso>(L, nblocks, stream); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <bool kX>
 hipError_t launch_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
                         hipStream_t stream) {
-  if (t.rows == 2 && t.waves == 4) return launch_batch_rw<2, 4, kX>(t.blt, mk, L, nblocks, stream);
-  if (t.rows == 2 && t.waves == 8) return launch_batch_rw<2, 8, kX>(t.blt, mk, L, nblocks, stream);
-  if (t.rows == 4 && t.waves == 4) return launch_batch_rw<4, 4, kX>(t.blt, mk, L, nblocks, stream);
-  return launch_batch_rw<4, 8, kX>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 2 && t.waves == 4) return launch_batch_rw<2, 4>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 2 && t.waves == 8) return launch_batch_rw<2, 8>(t.blt, mk, L, nblocks, stream);
+  if (t.rows == 4 && t.waves == 4) return launch_batch_rw<4, 4>(t.blt, mk, L, nblocks, stream);
+  return launch_batch_rw<4, 8>(t.blt, mk, L, nblocks, stream);
 }
 
 template <int B, int R, int W, int kMat, class S>
@@ -734,16 +753,15 @@ bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk) {
   return t.rows * t.waves - 2 * t.blt >= 1 && 64 - 2 * t.blt >= 1;
 }
 
-hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, bool lean, const BatchLaunch& L,
-                                int nblocks, hipStream_t stream) {
+hipError_t launch_zstream_batch(const ZTiling& t, MatKind mk, const BatchLaunch& L, int nblocks,
+                                hipStream_t stream) {
   if (!ztiling_supported(t, kF32, mk) || mk == kMatFull || nblocks < 1)
     return hipErrorInvalidValue;
-  return lean ? launch_batch<false>(t, mk, L, nblocks, stream)
-              : launch_batch<true>(t, mk, L, nblocks, stream);
+  return launch_batch(t, mk, L, nblocks, stream);
 }
 
-int zstream_batch_blocks(const ZTiling& t, MatKind mk, bool lean, int device) {
-  const void* fn = lean ? batch_kernel<false>(t, mk) : batch_kernel<true>(t, mk);
+int zstream_batch_blocks(const ZTiling& t, MatKind mk, int device) {
+  const void* fn = batch_kernel(t, mk);
   if (!fn) return 0;
   int per_cu = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * t.waves, 0) != hipSuccess)
