@@ -5,7 +5,7 @@ once, as the reference's --fmad=false build, CMakeLists.txt:25) and without
 packed-FP32 VALU ops. Objects go to gdiamond_amd/_build/, the library to
 gdiamond_amd/libgdiamond_hip.so.
 
-Usage: python -m gdiamond_amd.build   (or gdiamond_amd.build.build())
+Usage: python -m gdiamond_amd.build [--incremental]   (or gdiamond_amd.build.build())
 """
 from __future__ import annotations
 
@@ -43,9 +43,21 @@ def hipcc() -> str:
     return p
 
 
-def _compile(src: str) -> str:
+def _stale(obj: str, path: str) -> bool:
+    """True when obj is missing or older than its source or a shared header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps.append(os.path.join(ROOT, "include", "gdiamond", "gd_capi.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
+    if incremental and not _stale(obj, path):
+        return obj
     if src.endswith(".hip"):
         cmd = [hipcc(), *COMMON, *DEVICE, "-x", "hip", "-c", path, "-o", obj]
     else:
@@ -62,11 +74,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    """Compile every source and link the shared library; returns its path."""
+def build(verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every source (only the changed ones with incremental=True) and
+    link the shared library; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -79,4 +92,4 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
