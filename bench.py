@@ -284,63 +284,59 @@ def run_precision(gd, n=1024, steps=2, warmup=1, T=2000, error_leg=True):
 
 def run_learned(args, gd):
     """Learned form: 64 seeded fine/coarse solver pairs (fine 96^3, coarse
-    48^3, 8 snapshots each) train the corrector; a blend factor alpha for the
-    applied correction is chosen on a held-out problem (fine 128^3 x 400
-    steps); then the fine 512^3 x 2000-step run (eps 1-6, seed 6000) is the
-    reference for the coarse 256^3 x 1000-step run with and without the corrector
-    (every 4 coarse steps): relative DFT-spectrum error of Ex, Ey on the mid
-    plane and run time."""
+    48^3, 8 snapshots each) train the corrector; a second pass fine-tunes it on
+    rollout samples (the corrected coarse solver's own trajectories, 8
+    intervals from a restricted fine state). Then the fine 512^3 x 2000-step
+    run (eps 1-6, seed 6000) is the reference for the coarse 256^3 x
+    1000-step run without the corrector, with either corrector (applied in
+    full every 4 coarse steps), and with the exact correction (the restricted
+    fine state itself every 4 coarse steps: the bound of any corrector
+    applied that often). Relative DFT-spectrum errors of Ex, Ey on the mid
+    plane per frequency and per eps bin of the coarse plane, and run times."""
     from gdiamond_amd import learned
+    say = lambda s: print("# " + s, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    raw = learned.make_samples(npairs=args.learned_pairs, n=96, nsnap=8,
-                               log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+    raw = learned.make_samples(npairs=args.learned_pairs, n=96, nsnap=8, log=say)
     gen_s = time.perf_counter() - t0
     dx_over_dt = float(np.float32(2.0) / (np.float32(2.0) * learned.DT_FINE))
     cfg = learned.fit_scales(raw, dx_over_dt)
     base = learned.baseline_loss(raw, cfg)
     t0 = time.perf_counter()
-    params, hist = learned.train(raw, cfg, epochs=args.learned_epochs, lr=1e-3, lr_min=1e-4,
-                                 log=lambda s: print("# " + s, file=sys.stderr, flush=True))
+    params1, hist1 = learned.train(raw, cfg, epochs=args.learned_epochs, lr=1e-3, lr_min=1e-4,
+                                   log=say)
     train_s = time.perf_counter() - t0
-    del raw
+    t0 = time.perf_counter()
+    roll = learned.make_rollout_samples(params1, cfg, npairs=args.learned_pairs, n=96, log=say)
+    roll_base = learned.baseline_loss(roll, cfg)
+    params2, hist2 = learned.train(raw + roll, cfg, epochs=args.learned_epochs, lr=3e-4,
+                                   lr_min=3e-5, params0=params1, log=say)
+    rollout_s = time.perf_counter() - t0
+    del raw, roll
     if args.learned_params_out:
         d = os.path.dirname(os.path.abspath(args.learned_params_out))
         os.makedirs(d, exist_ok=True)
-        np.savez(args.learned_params_out, params=params, cfg=np.asarray(cfg))
+        np.savez(args.learned_params_out, params=params2, cfg=np.asarray(cfg))
 
-    def with_alpha(a):
-        c = list(cfg)
-        c[3] = cfg[3] / a
-        c[4] = cfg[4] / a
-        return tuple(c)
-
-    # blend factor on a held-out problem with the full rollout length (the
-    # same number of corrections as the measured run); alpha 0 = corrector off
-    val = {}
-    nval = 128
-    for a in (1.0 / 32, 1.0 / 16, 1.0 / 8, 1.0 / 4, 1.0 / 2, 1.0):
-        r = learned.compare(nval, args.timesteps * 2, params, with_alpha(a), seed=6100,
-                            nspheres=int(200 * (nval / 512) ** 3) + 1)
-        val[a] = float(np.max(r["learned_err"]))
-        val[0.0] = float(np.max(r["coarse_err"]))
-        print(f"# validation alpha {a}: learned {val[a]:.4f} coarse {val[0.0]:.4f}",
-              file=sys.stderr, flush=True)
-    alpha = min(val, key=lambda a: val[a] if np.isfinite(val[a]) else np.inf)
     n = args.size
     T = args.timesteps * 2  # fine steps (2000 at the default 1000)
-    res = learned.compare(n, T, params, with_alpha(alpha) if alpha > 0 else with_alpha(1.0 / 32),
-                          seed=6000, nspheres=200)
-    if alpha == 0.0:  # the corrector is off: the plain coarse run is the learned form
-        res["learned_err"], res["learned_ms"] = res["coarse_err"], res["coarse_ms"]
-    err_learned = float(np.max(res["learned_err"]))
-    err_coarse = float(np.max(res["coarse_err"]))
-    # the form that meets the error bound: the fine solver stays the fallback
-    chosen = "learned-coarse" if err_learned <= ERROR_BOUND else "fine (fallback)"
+    r1 = learned.compare(n, T, params1, cfg, seed=6000, nspheres=200, log=say)
+    r2 = learned.compare(n, T, params2, cfg, seed=6000, nspheres=200, log=say)
+    ideal = learned.ideal_correction(n, T, seed=6000, nspheres=200, log=say)
+
+    def worst(e):
+        v = np.asarray(e, np.float64)
+        return float(np.max(v)) if np.all(np.isfinite(v)) else float("inf")
+
+    forms = {"coarse": worst(r1["coarse_err"]), "learned": worst(r1["learned_err"]),
+             "learned_rollout": worst(r2["learned_err"]), "exact_correction": worst(ideal["ideal_err"])}
+    best = min(("learned", "learned_rollout"), key=lambda k: forms[k])
+    best_ms = r1["learned_ms"] if best == "learned" else r2["learned_ms"]
+    chosen = best if forms[best] <= ERROR_BOUND else "fine (fallback)"
     out = {
         "metric": "time-to-solution and DFT-spectrum error, learned coarse grid vs fine grid",
-        "value": round(res["fine_ms"] / res["learned_ms"], 3),
+        "value": round(r1["fine_ms"] / best_ms, 3),
         "unit": "x faster than the fine run",
-        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(res["learned_ms"], 2),
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best_ms, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
         "data": "synthetic",
         "config": {"workload": f"fdtd3d-yee-learned-{n}^3-{T}steps",
@@ -349,14 +345,25 @@ def run_learned(args, gd):
                    "network": "3x3x3 conv 12-16-16-12, BF16 MFMA, FP32 accumulate",
                    "training": {"pairs": args.learned_pairs, "fine": "96^3", "snapshots": 8,
                                 "epochs": args.learned_epochs, "gen_s": round(gen_s, 1),
-                                "train_s": round(train_s, 1), "loss_first": hist[0],
-                                "loss_last": hist[-1], "zero_correction_loss": base},
-                   "alpha_validation": {str(k): v for k, v in val.items()}, "alpha": alpha,
-                   "fine_ms": round(res["fine_ms"], 1), "coarse_ms": round(res["coarse_ms"], 1),
-                   "learned_ms": round(res["learned_ms"], 1),
-                   "spectrum_err_coarse": err_coarse, "spectrum_err_learned": err_learned,
-                   "error_bound": ERROR_BOUND, "chosen_at_bound": chosen,
-                   "errors": {k: res[k] for k in ("coarse_err", "learned_err")}},
+                                "train_s": round(train_s, 1), "rollout_s": round(rollout_s, 1),
+                                "loss_first": hist1[0], "loss_last": hist1[-1],
+                                "zero_correction_loss": base,
+                                "rollout_zero_correction_loss": roll_base,
+                                "rollout_loss_first": hist2[0], "rollout_loss_last": hist2[-1]},
+                   "fine_ms": round(r1["fine_ms"], 1), "coarse_ms": round(r1["coarse_ms"], 1),
+                   "learned_ms": round(r1["learned_ms"], 1),
+                   "learned_rollout_ms": round(r2["learned_ms"], 1),
+                   "spectrum_err_max": forms, "error_bound": ERROR_BOUND, "chosen_at_bound": chosen,
+                   "errors": {"frequencies_cycles_per_fine_step": [0.03, 0.05, 0.07],
+                              "components": ["Ex", "Ey"],
+                              "eps_bins": [list(b) for b in learned.EPS_BINS],
+                              "coarse": r1["coarse_err"], "coarse_eps_bins": r1["coarse_err_eps_bins"],
+                              "learned": r1["learned_err"],
+                              "learned_eps_bins": r1["learned_err_eps_bins"],
+                              "learned_rollout": r2["learned_err"],
+                              "learned_rollout_eps_bins": r2["learned_err_eps_bins"],
+                              "exact_correction": ideal["ideal_err"],
+                              "exact_correction_eps_bins": ideal["ideal_err_eps_bins"]}},
     }
     print(json.dumps(out), flush=True)
 

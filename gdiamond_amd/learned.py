@@ -159,6 +159,65 @@ def make_samples(npairs=64, n=96, nsnap=8, first=24, every=8, interval=4, seed0=
     return raw
 
 
+def apply_correction(net, cur, prev, cb, cfg):
+    """The corrector's step on host arrays, as gd_run_corrected does on the
+    device: features of (cur, prev E), network forward, F += y / r on interior
+    cells. Returns the corrected fields."""
+    net.set_input(features(cur, prev[:3], cb, cfg))
+    y = net.forward()
+    ie = np.float32(1.0) / np.float32(cfg[3])
+    ih = np.float32(1.0) / np.float32(cfg[4])
+    out = [f.copy() for f in cur]
+    for q in range(6):
+        s = ie if q < 3 else ih
+        out[q][1:-1, 1:-1, 1:-1] = out[q][1:-1, 1:-1, 1:-1] + y[1:-1, 1:-1, 1:-1, q] * s
+    return out
+
+
+def make_rollout_samples(params, cfg, npairs=64, n=96, start=24, nint=8, interval=4,
+                         seed0=5000, log=None):
+    """Training pairs along the corrected solver's own trajectory: from the
+    restricted fine state at coarse step `start`, the coarse solver runs nint
+    intervals with the corrector `params` applied after each; every interval
+    gives a sample (the coarse state before the correction and the restricted
+    fine state it should have been), so the network sees the errors its own
+    corrections leave behind."""
+    raw = []
+    for k in range(npairs):
+        seed = seed0 + k
+        f = float(np.random.default_rng(seed).uniform(0.02, 0.08))
+        last = start + nint * interval
+        prob = Problem(n, seed, 10, 4.0, 16.0, f, 2 * last + 4)
+        fine = prob.fine_ctx()
+        want = {}
+        for m in [start] + [start + q * interval + d for q in range(1, nint + 1) for d in (-1, 0)]:
+            want[m] = None
+        for m in sorted(want):
+            fine.run(2 * m - fine.step)
+            want[m] = restricted_state(fine, m)
+        fine.close()
+        cb = prob.cb_coarse()
+        nc = prob.nc
+        ctx = prob.coarse_ctx(step0=start)
+        net = Net(nc, nc, nc)
+        net.set_params(params)
+        ctx.set_fields(want[start])
+        for q in range(1, nint + 1):
+            t = start + q * interval
+            ctx.run(interval - 1)
+            prev = ctx.get_fields()
+            ctx.run(1)
+            cur = ctx.get_fields()
+            raw.append({"cur": cur, "prev": prev, "ref": want[t], "ref_prev": want[t - 1],
+                        "cb": cb})
+            ctx.set_fields(apply_correction(net, cur, prev, cb, cfg))
+        ctx.close()
+        net.close()
+        if log:
+            log(f"rollout pair {k}: seed {seed} f {f:.3f} samples {len(raw)}")
+    return raw
+
+
 def fit_scales(raw, dx_over_dt):
     def rms(vals):
         s = sum(float(np.mean(np.square(v, dtype=np.float64))) for v in vals)
@@ -186,13 +245,14 @@ def sample_tensors(r, cfg):
     return x, t, mask
 
 
-def train(raw, cfg, epochs=8, lr=1e-3, lr_min=1e-4, seed=0, log=None):
-    """Adam on all samples, cosine learning-rate decay; returns the parameters
-    and the mean loss of each epoch."""
+def train(raw, cfg, epochs=8, lr=1e-3, lr_min=1e-4, seed=0, log=None, params0=None):
+    """Adam on all samples, cosine learning-rate decay, from params0 (or a
+    seeded initialisation); returns the parameters and the mean loss of each
+    epoch."""
     nz, ny, nx = raw[0]["cb"].shape
     rng = np.random.default_rng(seed)
     net = Net(nx, ny, nz)
-    net.set_params(init_params(seed))
+    net.set_params(init_params(seed) if params0 is None else params0)
     tensors = [sample_tensors(r, cfg) for r in raw]
     steps = epochs * len(tensors)
     hist = []
@@ -233,11 +293,80 @@ def restrict_spectrum(re, im, axis_comp):
     return 0.5 * (X[..., 0::2, 0::2] + X[..., 1::2, 0::2])
 
 
+EPS_BINS = ((1.0, 1.0), (1.0, 3.0), (3.0, 6.0))  # vacuum, eps in (1,3], (3,6]
+
+
+def _errors(spec, ref, f, nc, eps_plane):
+    """Relative spectrum error per component and frequency over the plane, and
+    per component, frequency and eps bin (cells of the coarse plane grouped
+    by their eps)."""
+    errs, bins = [], []
+    masks = [(eps_plane == lo) if lo == hi else ((eps_plane > lo) & (eps_plane <= hi))
+             for lo, hi in EPS_BINS]
+    for X, R in zip(spec, ref):
+        R = R.reshape(len(f), nc, nc)
+        errs.append([float(np.linalg.norm(X[q] - R[q]) / np.linalg.norm(R[q]))
+                     for q in range(len(f))])
+        bins.append([[float(np.linalg.norm((X[q] - R[q])[m]) / np.linalg.norm(R[q][m]))
+                      if m.any() else None for m in masks] for q in range(len(f))])
+    return errs, bins
+
+
+def _coarse_spectra(ctx, f, nc):
+    # coarse sample m is fine time 2m+2, weighted as fine step 2m: shift
+    shift = np.exp(-2j * np.pi * np.asarray(f))[:, None, None]
+    spec = []
+    for c in (EX, EY):
+        re, im = ctx.dft_get(c)
+        spec.append((re + 1j * im).reshape(len(f), nc, nc) * shift)
+    return spec
+
+
+def ideal_correction(n_fine, nsteps_fine, seed=6000, nspheres=200, rmin=4.0, rmax=20.0,
+                     f_cps=(0.03, 0.05, 0.07), interval=4, log=None):
+    """The best any corrector applied every `interval` coarse steps can do:
+    the coarse run whose state is replaced, at every correction, by the
+    restricted fine state itself (an exact correction), against the fine
+    run's restricted spectrum. Errors as compare()."""
+    prob = Problem(n_fine, seed, nspheres, rmin, rmax, 0.05, nsteps_fine)
+    n, nc = prob.n, prob.nc
+    f = np.asarray(f_cps, np.float64)
+    fine = prob.fine_ctx()
+    fine.dft_config([EX, EY], axis=2, index=n // 2, f_cps=f)
+    ctx = prob.coarse_ctx()
+    ctx.dft_config([EX, EY], axis=2, index=nc // 2, f_cps=2 * f, scale=2.0)
+    t0 = time.perf_counter()
+    m, M = 0, nsteps_fine // 2
+    while m + interval <= M:
+        ctx.run(interval)
+        m += interval
+        if m < M:  # a correction after the last step changes no DFT sample
+            fine.run(2 * m - fine.step)
+            ctx.set_fields(restricted_state(fine, m))
+    if m < M:
+        ctx.run(M - m)
+    fine.run(nsteps_fine - fine.step)
+    secs = time.perf_counter() - t0
+    ref = []
+    for q, c in enumerate((EX, EY)):
+        re, im = fine.dft_get(c)
+        ref.append(restrict_spectrum(re.reshape(len(f), n, n), im.reshape(len(f), n, n), q))
+    fine.close()
+    spec = _coarse_spectra(ctx, f, nc)
+    ctx.close()
+    errs, bins = _errors(spec, ref, f, nc, np.asarray(prob.eps_c)[nc // 2])
+    out = {"ideal_err": errs, "ideal_err_eps_bins": bins, "seconds": secs}
+    if log:
+        log(str(out))
+    return out
+
+
 def compare(n_fine, nsteps_fine, params, cfg, seed=6000, nspheres=200, rmin=4.0, rmax=20.0,
             f_cps=(0.03, 0.05, 0.07), interval=4, log=None):
     """Fine run (the reference) against the coarse run without and with the
-    corrector: relative DFT-spectrum error of Ex and Ey on the mid plane
-    (max over components and frequencies) and wall time of each run."""
+    corrector: relative DFT-spectrum error of Ex and Ey on the mid plane per
+    component and frequency (and per eps bin of the coarse plane's cells),
+    and wall time of each run."""
     prob = Problem(n_fine, seed, nspheres, rmin, rmax, 0.05, nsteps_fine)
     n, nc = prob.n, prob.nc
     f = np.asarray(f_cps, np.float64)
@@ -267,25 +396,16 @@ def compare(n_fine, nsteps_fine, params, cfg, seed=6000, nspheres=200, rmin=4.0,
         else:
             ctx.run_corrected(net, nsteps_fine // 2, interval, cfg)
         ms = (time.perf_counter() - t0) * 1e3
-        # coarse sample m is fine time 2m+2, weighted as fine step 2m: shift
-        shift = np.exp(-2j * np.pi * f)[:, None, None]
-        spec = []
-        for c in (EX, EY):
-            re, im = ctx.dft_get(c)
-            spec.append((re + 1j * im).reshape(len(f), nc, nc) * shift)
+        spec = _coarse_spectra(ctx, f, nc)
         ctx.close()
         if net is not None:
             net.close()
-        errs = []
-        for X, R in zip(spec, ref):
-            R = R.reshape(len(f), nc, nc)
-            errs.append([float(np.linalg.norm(X[q] - R[q]) / np.linalg.norm(R[q]))
-                         for q in range(len(f))])
-        return ms, errs
+        errs, bins = _errors(spec, ref, f, nc, np.asarray(prob.eps_c)[nc // 2])
+        return ms, errs, bins
 
-    out["coarse_ms"], out["coarse_err"] = coarse(False)
+    out["coarse_ms"], out["coarse_err"], out["coarse_err_eps_bins"] = coarse(False)
     if params is not None:
-        out["learned_ms"], out["learned_err"] = coarse(True)
+        out["learned_ms"], out["learned_err"], out["learned_err_eps_bins"] = coarse(True)
     if log:
         log(str(out))
     return out

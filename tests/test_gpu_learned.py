@@ -180,3 +180,26 @@ def test_training_pipeline_small(gd):
     res = learned.compare(32, 120, params, cfg, seed=6000, nspheres=6, rmin=4.0, rmax=10.0,
                           log=say)
     assert np.isfinite(res["learned_err"]).all() and np.isfinite(res["coarse_err"]).all()
+
+
+def test_rollout_samples_and_exact_correction(gd):
+    """Rollout samples come from the corrected coarse trajectory (one per
+    interval, coarse state before the correction against the restricted fine
+    state); the exact correction (the restricted fine state every 4 coarse
+    steps) runs and reports finite errors per component, frequency and eps
+    bin. (It is not a bound: a restricted state is not a state of the coarse
+    scheme, and in vacuum the resets can raise the error.)"""
+    from gdiamond_amd import learned
+    raw = learned.make_samples(npairs=1, n=32, nsnap=2, first=12, every=4, log=say)
+    cfg = learned.fit_scales(raw, float(np.float32(2.0) / (np.float32(2.0) * learned.DT_FINE)))
+    params = learned.init_params(3)
+    roll = learned.make_rollout_samples(params, cfg, npairs=2, n=32, start=12, nint=3, log=say)
+    assert len(roll) == 6
+    for r in roll:
+        assert r["cur"][0].shape == (16, 16, 16) and np.isfinite(r["cur"][2]).all()
+        assert not np.array_equal(r["cur"][2], r["ref"][2])
+    ideal = learned.ideal_correction(32, 160, nspheres=6, rmin=4.0, rmax=10.0, log=say)
+    plain = learned.compare(32, 160, None, cfg, nspheres=6, rmin=4.0, rmax=10.0, log=say)
+    assert np.isfinite(ideal["ideal_err"]).all() and np.isfinite(plain["coarse_err"]).all()
+    assert np.asarray(ideal["ideal_err"]).shape == (2, 3)
+    assert len(ideal["ideal_err_eps_bins"]) == 2 and len(ideal["ideal_err_eps_bins"][0][0]) == 3
