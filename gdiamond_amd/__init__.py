@@ -23,7 +23,7 @@ EX, EY, EZ, HX, HY, HZ = range(6)
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 COEFF_NAMES = ("Cax", "Cbx", "Cay", "Cby", "Caz", "Cbz",
                "Dax", "Dbx", "Day", "Dby", "Daz", "Dbz")
-PLAIN, TILED = 0, 1
+PLAIN, TILED, FUSED = 0, 1, 2
 F32, F64, F16, BF16 = 0, 1, 2, 3  # storage formats
 MAT_AUTO, MAT_DENSE, MAT_INDEXED, MAT_FULL = 0, 1, 2, 3  # material formats
 BATCH_PERSISTENT, BATCH_GRAPH, BATCH_PER_INSTANCE = 0, 1, 2  # batch run modes

@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libgdiamond_hip.so")
 ARCH = os.environ.get("GD_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["gd_capi.hip", "gd_batch.hip", "gd_plain.hip", "gd_zstream.hip", "gd_learned.hip",
-           "gd_noalloc.hip", "gd_host.cpp"]
+           "gd_noalloc.hip", "gd_fused.hip", "gd_host.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -967,7 +967,7 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
 
 int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
   if (!c) return GD_ERR_ARG;
-  if (variant != GD_VARIANT_PLAIN && variant != GD_VARIANT_TILED)
+  if (variant != GD_VARIANT_PLAIN && variant != GD_VARIANT_TILED && variant != GD_VARIANT_FUSED)
     return c->fail(GD_ERR_ARG, "unknown variant");
   if (!c->have_mat) return c->fail(GD_ERR_STATE, "no material set");
   if (c->step + nsteps > (size_t)INT32_MAX) return c->fail(GD_ERR_ARG, "step count too large");
@@ -1000,6 +1000,15 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       hipError_t e = hipMalloc(&c->dft_snap, n * sizeof(float));
       if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(dft snapshot)");
     }
+  }
+
+  if (variant == GD_VARIANT_FUSED) {
+    if (c->storage == GD_STORAGE_F64 && c->mat_kind == gd::kMatIdx)
+      return c->fail(GD_ERR_UNSUPPORTED, "FP64 storage takes the dense material");
+    if (c->nx * c->ny * 8 >= (size_t(1) << 32))
+      return c->fail(GD_ERR_UNSUPPORTED, "the fused kernel needs an xy plane below 512 Mi cells");
+    int rc = alloc_fields(c, 1);
+    if (rc != GD_OK) return rc;
   }
 
   gd::Grid g;
@@ -1097,6 +1106,21 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
                                       (int)r, c->stream));
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
     }
+  } else if (variant == GD_VARIANT_FUSED) {
+    for (size_t r = 0; r < nsteps; ++r) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (c->launch_timing) {
+        a = pool_event(c, nev++);
+        b = pool_event(c, nev++);
+        if (!a || !b) return c->fail(GD_ERR_HIP, "event creation failed");
+        GD_HIP(c, hipEventRecord(a, c->stream));
+      }
+      GD_HIP(c, gd::launch_fused_step(st, fields_of(c, c->cur), fields_of(c, 1 - c->cur), sc, m,
+                                      c->mat_kind, g, s, d, (int)(c->step + r), (int)r,
+                                      c->stream));
+      if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
+      c->cur = 1 - c->cur;
+    }
   } else {
     // With the planner's 4 x 8 tile a launch is split: the few tiles that hold
     // a source cell in their window or own monitored cells run the full
@@ -1188,7 +1212,7 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
       sum += t;
     }
     c->last_kernel_ms = sum;
-    c->last_nlaunch = variant == GD_VARIANT_PLAIN ? nsteps * 2 : nev / 2;
+    c->last_nlaunch = variant == GD_VARIANT_PLAIN ? nsteps * 2 : nev / 2;  // PLAIN: E and H
   }
   c->step += nsteps;
   return GD_OK;

@@ -335,6 +335,62 @@ __device__ __forceinline__ void source_terms(const Sources& s, int base_comp, in
   }
 }
 
+// ---- device helpers shared by the stepping kernels
+
+// Lane l receives the value of lane l-1 (lane 0 receives 0): wave_shr:1.
+// bound_ctrl writes 0 for the lane without a source, which lets the compiler
+// fold the shift into the subtraction that consumes it (v_sub*_f32_dpp).
+__device__ __forceinline__ float from_lane_below(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
+}
+// Lane l receives the value of lane l+1 (lane 63 receives 0): wave_shl:1.
+__device__ __forceinline__ float from_lane_above(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
+}
+// The same shifts for doubles, one DPP move per 32-bit half.
+template <int kCtrl>
+__device__ __forceinline__ double shift_lanes_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, 0xF, 0xF,
+                                                            true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl,
+                                                            0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double from_lane_below(double v) { return shift_lanes_f64<0x138>(v); }
+__device__ __forceinline__ double from_lane_above(double v) { return shift_lanes_f64<0x130>(v); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs. Give each XCD a
+// contiguous run of tiles (x fastest, then y, then z) so tiles that share
+// halos share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Load / store at a per-plane base (uniform, SGPRs) plus a 32-bit byte offset
+// (per lane), so the hardware's scalar-base + vector-offset addressing is used
+// instead of 64-bit per-lane address arithmetic. The offsets are computed
+// once before the stream loop; without the empty asm the compiler carries
+// them into the loop as 64-bit values and no longer sees that the high half
+// is zero, so every 4- or 8-byte access in the loop would cost a 64-bit add.
+// (For 2-byte elements the plain form measures faster.)
+template <class S>
+__device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
+  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
+  return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
+}
+template <class S>
+__device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
+  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
+  *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
+}
+
 }  // namespace gd
 
 struct gd_nn;
@@ -343,6 +399,10 @@ namespace gd {
 hipError_t launch_plain_step(Storage st, const Fields& f, const Scales& sc, const Material& m,
                              MatKind mk, const Grid& g, const Sources& s, const Dft& d, int n,
                              int run_step, hipStream_t stream);
+// One step of the FUSED kernel: fields of step n from `in`, step n+1 to `out`.
+hipError_t launch_fused_step(Storage st, const Fields& in, const Fields& out, const Scales& sc,
+                             const Material& m, MatKind mk, const Grid& g, const Sources& s,
+                             const Dft& d, int n, int run_step, hipStream_t stream);
 
 struct ZTiling {
   int blt, rows, waves, zchunk;

@@ -50,48 +50,6 @@ namespace gd {
 
 namespace {
 
-// Lane l receives the value of lane l-1 (lane 0 receives 0): wave_shr:1.
-// bound_ctrl writes 0 for the lane without a source, which lets the compiler
-// fold the shift into the subtraction that consumes it (v_sub*_f32_dpp).
-__device__ __forceinline__ float from_lane_below(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
-}
-// Lane l receives the value of lane l+1 (lane 63 receives 0): wave_shl:1.
-__device__ __forceinline__ float from_lane_above(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs. Give each XCD a
-// contiguous run of tiles (x fastest, then y, then z) so tiles that share
-// halos share that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int q = n >> 3, r = n & 7;
-  const int x = b & 7, i = b >> 3;
-  return x * q + min(x, r) + i;
-}
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
-
-// Load / store at a per-plane base (uniform, SGPRs) plus a 32-bit byte offset
-// (per lane), so the hardware's scalar-base + vector-offset addressing is used
-// instead of 64-bit per-lane address arithmetic. The offsets are computed
-// once before the stream loop; without the empty asm the compiler carries
-// them into the loop as 64-bit values and no longer sees that the high half
-// is zero, so every 4- or 8-byte access in the loop would cost a 64-bit add.
-// (For 2-byte elements the plain form measures faster.)
-template <class S>
-__device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
-  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
-  return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
-}
-template <class S>
-__device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
-  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
-  *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
-}
-
 // Subtract the point-source terms of step n on plane z from the curl
 // brackets c[0..2] (components base..base+2) of row gy (uniform) at column gx.
 // The loop over sources is uniform; a lane applies a term only on

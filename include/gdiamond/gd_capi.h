@@ -59,8 +59,15 @@ enum { GD_EX = 0, GD_EY = 1, GD_EZ = 2, GD_HX = 3, GD_HY = 4, GD_HZ = 5 };
  *   copies. Replaces the mix-mapping kernels ver2/ver3
  *   (kernels_mm_ver2.cuh:488-720, kernels_mm_ver3.cuh:417-647) behind
  *   update_FDTD_mix_mapping_gpu_ver2/ver3 (gdiamond_gpu_mm_ver2.cuh:837,
- *   gdiamond_gpu_mm_ver3.cuh:478). Results are bit-identical to PLAIN. */
-enum { GD_VARIANT_PLAIN = 0, GD_VARIANT_TILED = 1 };
+ *   gdiamond_gpu_mm_ver3.cuh:478). Results are bit-identical to PLAIN.
+ * GD_VARIANT_FUSED: one launch per step that updates E and then H in one
+ *   pass along z, reading step n from one field copy and writing step n+1
+ *   to the other (52 bytes per cell-update for FP32 against PLAIN's 76;
+ *   twice the field memory). The same update as PLAIN
+ *   (update_FDTD_gpu_3D_warp_underutilization_fix, gdiamond_gpu.cuh:896-913),
+ *   bit for bit, in every storage format (16-bit formats round every step,
+ *   as PLAIN does). */
+enum { GD_VARIANT_PLAIN = 0, GD_VARIANT_TILED = 1, GD_VARIANT_FUSED = 2 };
 
 /* Tiling of GD_VARIANT_TILED. Zero fields select the planner's choice.
  *  blt    : time steps per launch (1..4)
