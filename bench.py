@@ -47,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # the monitored plane; launch times span both
 TILED_KERNELS = "zstream_lean_kernel + zstream_kernel (concurrent pair per launch)"
 PLAIN_KERNELS = "plain_e_kernel + plain_h_kernel (one pair per step)"
+FUSED_KERNELS = "fused_step_kernel (one launch per step)"
 # the headline metric of the 512^3 workload
 METRIC = "Gcell-updates/s at 512\u00b3 + % HBM roofline; chosen vs plain speedup at fixed error"
 
@@ -159,6 +160,7 @@ def error_leg(gd, n, T, eps, dt, src, f_cps, tiled_spec, tiled_ms):
     print(f"# fp64 plain reference {ref_ms:.0f} ms", file=sys.stderr, flush=True)
     forms = {"f32-tiled": (tiled_spec, tiled_ms)}
     forms["f32-plain"] = one(gd.F32, gd.PLAIN)
+    forms["f32-fused"] = one(gd.F32, gd.FUSED)
     forms["f16-tiled"] = one(gd.F16, gd.TILED, [F16_SCALE_EXP] * 6)
     rep = {}
     for name, (spec, ms) in forms.items():
@@ -480,7 +482,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--timesteps", type=int, default=1000)
-    ap.add_argument("--variant", choices=["tiled", "plain"], default="tiled")
+    ap.add_argument("--variant", choices=["tiled", "plain", "fused"], default="tiled")
     ap.add_argument("--storage", choices=["f32", "f16"], default="f32",
                     help="field storage: FP32, or binary16 with scale 2^%d" % F16_SCALE_EXP)
     ap.add_argument("--material", choices=["auto", "dense", "indexed"], default="auto",
@@ -557,7 +559,7 @@ def main():
     f_cps = np.array([0.03, 0.05, 0.07])
     ctx.dft_config([gd.EX, gd.EZ], axis=2, index=c, f_cps=f_cps)
     ctx.set_launch_timing(True)
-    variant = gd.TILED if args.variant == "tiled" else gd.PLAIN
+    variant = {"tiled": gd.TILED, "plain": gd.PLAIN, "fused": gd.FUSED}[args.variant]
     tiling = None
     if variant == gd.TILED:
         tiling = ctx.plan_tiling()
@@ -632,12 +634,13 @@ def main():
         want = {tiling_line(tiling), f"storage: {args.storage}", f"material: {material}",
                 f"grid: {n}^3"}
     else:
-        traffic_name = f"plain_{n}cube_{args.storage}_{material}_traffic.txt"
+        traffic_name = f"{args.variant}_{n}cube_{args.storage}_{material}_traffic.txt"
         want = {f"storage: {args.storage}", f"material: {material}", f"grid: {n}^3"}
     traffic = parse_traffic(os.path.join(ROOT, "profiles", traffic_name), want)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": TILED_KERNELS if variant == gd.TILED else PLAIN_KERNELS,
+                "kernel": {gd.TILED: TILED_KERNELS, gd.PLAIN: PLAIN_KERNELS,
+                           gd.FUSED: FUSED_KERNELS}[variant],
                 "bytes_alg_per_cell_update": b_alg,
                 "mean_launch_ms": round(mean_launch_ms, 4),
                 "launches_per_run": nlaunch,

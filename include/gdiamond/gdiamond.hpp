@@ -11,7 +11,10 @@
 // Engine mapping:
 //   update_FDTD_seq / update_FDTD_omp             host loops (1 / N threads)
 //   update_FDTD_gpu_2D                            GD_VARIANT_PLAIN, per-step source
-//   update_FDTD_gpu_3D_warp_underutilization_fix  GD_VARIANT_PLAIN, constant source
+//   update_FDTD_gpu_3D_warp_underutilization_fix  GD_VARIANT_FUSED, constant source
+//                                                 (the same update as the
+//                                                 reference's E/H kernel pair,
+//                                                 fused into one launch)
 //   update_FDTD_mix_mapping_gpu_ver2 / _ver3      GD_VARIANT_TILED, constant source
 //   update_FDTD_gpu_figures                       GD_VARIANT_PLAIN, Gaussian Jx source,
 //                                                 FP64 DFT of Ex on the k = Nz/2 plane
@@ -96,7 +99,7 @@ class gDiamond {
     gpu_engine(num_timesteps, GD_VARIANT_PLAIN, per_step_mz(num_timesteps), _gpu);
   }
   void update_FDTD_gpu_3D_warp_underutilization_fix(size_t num_timesteps) {
-    gpu_engine(num_timesteps, GD_VARIANT_PLAIN, constant_mz(num_timesteps), _gpu_bl);
+    gpu_engine(num_timesteps, GD_VARIANT_FUSED, constant_mz(num_timesteps), _gpu_bl);
   }
   void update_FDTD_mix_mapping_gpu_ver2(size_t num_timesteps, size_t, size_t, size_t) {
     gpu_engine(num_timesteps, GD_VARIANT_TILED, constant_mz(num_timesteps), _gpu);
