@@ -327,7 +327,7 @@ def test_config_spheres_512_full_run_vs_oracle(gpu, oracle):
     """The 512^3 sphere workload (200 spheres, r 4-20, eps 1-12, seed 42,
     Gaussian Jz at the centre, FP64 DFT of Ex and Ez on z = 256 at
     0.03/0.05/0.07 cycles/step) for all 1000 steps: TILED (the planner's
-    tiling, the benchmark's material form) and PLAIN each equal the oracle's
+    tiling, the benchmark's material form), PLAIN and FUSED each equal the oracle's
     OpenMP update of the whole run on every element of the six fields and of
     the DFT sums, bit for bit -- the reference's whole-run relation
     (unittests/checker_gpu.cu:9-22, gdiamond.hpp:1330-1347)."""
@@ -341,7 +341,7 @@ def test_config_spheres_512_full_run_vs_oracle(gpu, oracle):
     setup = dict(cb=cb, db=db, sources=[(gpu.EZ, *centre(shape), src)],
                  dft=dict(comps=[0, 2], axis=2, index=256, f_cps=f_cps))
     got = {}
-    for variant in (gpu.TILED, gpu.PLAIN):
+    for variant in (gpu.TILED, gpu.PLAIN, gpu.FUSED):
         say(f"[512 x {T}] variant {variant}")
         got[variant] = run_variant(gpu, shape, setup, T, variant)
     # the oracle in chunks of 100 steps (16 threads), continuing its DFT sums
