@@ -2,11 +2,11 @@
 vs the oracle and the reference's golden vectors (GPU), bit for bit.
 
 The fused kernel's decomposition has its own edges: 62-cell x segments (lanes
-1..62 of a wave), 16-row groups (4 waves x 4 rows, each wave recomputing E on
-the row above its rows), 32-plane z chunks (E recomputed on the plane after
-the chunk, H of the grid's last plane copied after the loop). The shapes and
-source cells below sit on and next to each of those edges; every result must
-equal the sequential update, as PLAIN's does.
+1..62 of a wave), rows owned one per wave in 8-row workgroups (each wave
+recomputing E on the row above its own), 32-plane z chunks (E recomputed on
+the plane after the chunk, H of the grid's last plane copied after the
+loop). The shapes and source cells below sit on and next to each of those
+edges; every result must equal the sequential update, as PLAIN's does.
 """
 import numpy as np
 import pytest
@@ -34,9 +34,9 @@ def test_fused_reference_default_problem_golden(gpu, name):
     assert_fields_equal(got, [g[c] for c in gpu.COMPONENTS], f"fused {name}")
 
 
-# x: 62-cell segments; y: 16-row groups; z: 32-plane chunks
+# x: 62-cell segments; y: 8-row workgroups of one-row waves; z: 32-plane chunks
 EDGE_SHAPES = [(5, 4, 3), (62, 16, 32), (63, 17, 33), (125, 33, 65), (124, 47, 64), (70, 45, 33),
-               (3, 3, 3)]
+               (3, 3, 3), (62, 8, 32), (63, 9, 31)]
 
 
 @pytest.mark.parametrize("shape", EDGE_SHAPES)
@@ -48,8 +48,8 @@ def test_fused_iso_sphere_medium_edges(gpu, oracle, shape):
     eps, cb, db = iso_problem(gpu, shape, seed=11, nspheres=6, rmin=2, rmax=5)
     T = 17
     cells = {(min(i, nx - 1), min(j, ny - 1), min(k, nz - 1))
-             for i, j, k in [centre(shape), (61, 15, 31), (62, 16, 32), (1, 1, 1), (60, 14, 30),
-                             (nx - 2, ny - 2, nz - 2), (nx - 1, 0, nz // 2)]}
+             for i, j, k in [centre(shape), (61, 15, 31), (62, 16, 32), (1, 1, 1), (61, 7, 31),
+                             (62, 8, 32), (nx - 2, ny - 2, nz - 2), (nx - 1, 0, nz // 2)]}
     src = [((q % 6), *c, gpu.gaussian_pulse_table(T, 5.0, 3.0, 0.05 + 0.01 * q))
            for q, c in enumerate(sorted(cells))]
     setup = dict(cb=cb, db=db, sources=src, fields=random_fields(shape, 3, scale=0.05),
