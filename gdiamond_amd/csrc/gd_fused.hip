@@ -31,12 +31,17 @@ namespace gd {
 
 namespace {
 
-// One row per wave, 8 waves per workgroup: at 512^3 FP32 65.4 Gcell/s against
-// 61.2 with 4 rows x 4 waves (2 x 8: 61.8; 1 x 16: 65.5), binary16 equal;
-// the halo rows a wave reads are its neighbours' own rows, found in cache.
-constexpr int kFuW = 8;    // waves per workgroup
-constexpr int kFuR = 1;    // owned rows per wave
-constexpr int kFuKc = 32;  // z planes per wave
+// Three rows per wave, one wave per workgroup, 8-plane z chunks: many short,
+// light waves whose halo rows and halo planes are their neighbours' own,
+// found in cache. At 512^3 (FP32 / binary16 / FP64 Gcell/s, rows x waves x
+// planes, measured on two boxes): 3 x 1 x 8 77.3 / 131 / 36.5 (here);
+// 3 x 1 x 7 76.3 / 128; 3 x 1 x 9 74.8 / 130; 3 x 2 x 8 75.0 / 126;
+// 2 x 1 x 8 75.7 / 125; 4 x 1 x 8 76.8 / 124; 5 x 1 x 8 75.8 / 89;
+// 2 x 2 x 8 74.6 / 128; 1 x 4 x 8 70.8 / 123; 1 x 8 x 32 65.4 / 107;
+// 4 x 4 x 32 61.2 / 106; 1 x 8 x 64 55.1 / 103 (PLAIN: 58.4 / 63 / 30.3).
+constexpr int kFuW = 1;    // waves per workgroup
+constexpr int kFuR = 3;    // owned rows per wave
+constexpr int kFuKc = 8;   // z planes per wave
 constexpr int kFuOx = 62;  // owned x cells per wave (lanes 1..62)
 
 template <class S, int kMat>
@@ -152,15 +157,10 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
     hy_pa = fld(4, pz, cell_a);
   }
 
-  // Inputs of a plane in the storage format: own rows, Hz and Hx of the row
-  // below, Ex Ez Hx Hy Hz of the row above, and the material id of the own
-  // rows and the row above (isotropic material forms). 16-bit fields are
-  // loaded one plane ahead, so that their latency overlaps the updates of the
-  // current plane (512^3: 106 against 77 Gcell/s); 4- and 8-byte fields are
-  // loaded at the top of their own iteration and stored where they are
-  // computed: the registers a prefetch would hold cost more occupancy than
-  // the overlap gains (FP32: 66.8 against 64.5 Gcell/s).
-  constexpr bool kPf = sizeof(S) < 4;
+  // Inputs of a plane in the storage format, loaded one plane ahead so that
+  // their latency overlaps the updates of the current plane: own rows, Hz and
+  // Hx of the row below, Ex Ez Hx Hy Hz of the row above, and the material id
+  // of the own rows and the row above (isotropic material forms).
   using MT = typename std::conditional<kIdx, uint8_t, T>::type;
   S pe[R][3], ph[R][3], p_hzb, p_hxb, p_exa, p_eza, p_hxa, p_hya, p_hza;
   MT pm[R], pma;
@@ -187,10 +187,9 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       pma = ldp<MT>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell_a * MS);
     }
   };
-  if constexpr (kPf) prefetch(k0);
+  prefetch(k0);
 
   for (int k = k0; k <= k_end; ++k) {
-    if constexpr (!kPf) prefetch(k);
     const size_t pz = (size_t)k * plane_cells;
     const bool intz = k >= 1 && k <= nz - 2;
     T e[R][3], h[R][3];
@@ -211,13 +210,12 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       for (int r = 0; r < R; ++r) cbr[r] = kIdx ? (T)lds_cb[(unsigned)pm[r]] : (T)pm[r];
       cba = kIdx ? (T)lds_cb[(unsigned)pma] : (T)pma;
     }
-    // With the prefetch, the stores of the previous iteration (E of plane
-    // k-1, H of plane k-2) are issued here, before the next plane's loads:
-    // the vector-memory counter completes in issue order, so the wait for
-    // those loads one iteration from now finds them long done instead of
-    // waiting for stores just issued. Without it they are issued where the
-    // values are computed.
-    if (kPf && k - 1 >= k0) {
+    // The stores of the previous iteration (E of plane k-1, H of plane k-2)
+    // are issued here, before the next plane's loads: the vector-memory
+    // counter completes in issue order, so the wait for those loads one
+    // iteration from now finds them long done instead of waiting for stores
+    // just issued.
+    if (k - 1 >= k0) {
       const size_t pe_off = (size_t)(k - 1) * plane_cells * ES;
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -237,9 +235,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
           }
       }
     }
-    if constexpr (kPf) {
-      if (k < k_end) prefetch(k + 1);
-    }
+    if (k < k_end) prefetch(k + 1);
 
     // ---- E of step n+1 on plane k, own rows
     T en[R][3];
@@ -304,17 +300,9 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       eza_n = C::round(eza_n, sc.s[2], sc.inv[2]);
     }
 
-    // ---- owned E of plane k (the plane past the chunk belongs to the next)
+    // ---- DFT of the owned E of plane k (the plane past the chunk belongs to
+    // the next); the E values are stored at the top of the next iteration
     if (k < k1) {
-      if constexpr (!kPf) {
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-          if (own[r]) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              stp<S>(out.c[c], pz * ES, cell[r] * ES, C::enc(en[r][c], sc.inv[c]));
-          }
-      }
       if (dft_e) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -361,14 +349,8 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
           hn[1] = u ? hp[r][1] + db * cy : hp[r][1];
           hn[2] = u ? hp[r][2] + db * cz : hp[r][2];
         }
-        if constexpr (kPf) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) hs[r][c] = hn[c];
-        } else if (own[r]) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            stp<S>(out.c[3 + c], ph * ES, cell[r] * ES, C::enc(hn[c], sc.inv[3 + c]));
-        }
+        for (int c = 0; c < 3; ++c) hs[r][c] = hn[c];
         if (dft_h) {
           const long long p = own[r] ? dft_plane_index(d, g, i, j0 + r, kh) : -1;
           if (p >= 0)
@@ -391,9 +373,9 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
     ez_pa = eza_n;
   }
 
-  // With the prefetch: stores of the last iteration, E of plane k_end when it
-  // is owned, H of plane k_end-1.
-  if (kPf && k_end < k1) {
+  // Stores of the last iteration: E of plane k_end when it is owned, H of
+  // plane k_end-1.
+  if (k_end < k1) {
     const size_t pe_off = (size_t)k_end * plane_cells * ES;
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -403,7 +385,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
           stp<S>(out.c[c], pe_off, cell[r] * ES, C::enc(ep[r][c], sc.inv[c]));
       }
   }
-  if (kPf && k_end - 1 >= k0) {
+  if (k_end - 1 >= k0) {
     const size_t ph_off = (size_t)(k_end - 1) * plane_cells * ES;
 #pragma unroll
     for (int r = 0; r < R; ++r)

@@ -2,11 +2,11 @@
 vs the oracle and the reference's golden vectors (GPU), bit for bit.
 
 The fused kernel's decomposition has its own edges: 62-cell x segments (lanes
-1..62 of a wave), rows owned one per wave in 8-row workgroups (each wave
-recomputing E on the row above its own), 32-plane z chunks (E recomputed on
-the plane after the chunk, H of the grid's last plane copied after the
-loop). The shapes and source cells below sit on and next to each of those
-edges; every result must equal the sequential update, as PLAIN's does.
+1..62 of a wave), 3 rows per wave (each wave recomputing E on the row above
+its rows), 8-plane z chunks (E recomputed on the plane after the chunk, H of
+the grid's last plane copied after the loop). The shapes and source cells
+below sit on and next to each of those edges; every result must equal the
+sequential update, as PLAIN's does.
 """
 import numpy as np
 import pytest
@@ -34,7 +34,7 @@ def test_fused_reference_default_problem_golden(gpu, name):
     assert_fields_equal(got, [g[c] for c in gpu.COMPONENTS], f"fused {name}")
 
 
-# x: 62-cell segments; y: 8-row workgroups of one-row waves; z: 32-plane chunks
+# x: 62-cell segments; y: 3 rows per wave; z: 8-plane chunks
 EDGE_SHAPES = [(5, 4, 3), (62, 16, 32), (63, 17, 33), (125, 33, 65), (124, 47, 64), (70, 45, 33),
                (3, 3, 3), (62, 8, 32), (63, 9, 31)]
 
