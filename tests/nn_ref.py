@@ -69,4 +69,4 @@ def loss_and_grads(params, x, target, mask):
     for w, b in ws:
         gw = w.grad.permute(2, 3, 4, 0, 1).reshape(-1).numpy()  # [dz][dy][dx][m][k]
         g.append(np.concatenate([gw, b.grad.numpy()]))
-    return float(loss), np.concatenate(g).astype(np.float32)
+    return float(loss.detach()), np.concatenate(g).astype(np.float32)
