@@ -108,6 +108,8 @@ _SIGS = {
     "gd_dft_get": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP]),
     "gd_dft_disable": (C.c_int, [C.c_void_p]),
     "gd_run": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.POINTER(Tiling)]),
+    "gd_run_async": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.POINTER(Tiling)]),
+    "gd_sync": (C.c_int, [C.c_void_p]),
     "gd_plan_tiling": (C.c_int, [C.c_void_p, C.POINTER(Tiling)]),
     "gd_plan_tiling_for": (C.c_int, [C.c_size_t, C.c_size_t, C.c_size_t, C.c_int,
                                      C.POINTER(Tiling)]),
@@ -442,12 +444,19 @@ class Context:
         self._dft = None
 
     # -- stepping
-    def run(self, nsteps, variant=TILED, tiling=None):
+    def run(self, nsteps, variant=TILED, tiling=None, wait=True):
+        """Advance nsteps steps. wait=False enqueues the run and returns at
+        once (gd_run_async); sync() or the next call that reads or changes the
+        context completes it."""
         t = None
         if tiling is not None:
             t = Tiling(**tiling) if isinstance(tiling, dict) else tiling
-        self._ok(lib().gd_run(self._h, int(variant), int(nsteps),
-                              C.byref(t) if t is not None else None))
+        fn = lib().gd_run if wait else lib().gd_run_async
+        self._ok(fn(self._h, int(variant), int(nsteps), C.byref(t) if t is not None else None))
+
+    def sync(self):
+        """Waits for a run enqueued with run(..., wait=False)."""
+        self._ok(lib().gd_sync(self._h))
 
     def plan_tiling(self):
         t = Tiling()

@@ -69,6 +69,13 @@ struct gd_ctx {
   float* dft_snap = nullptr;  // TILED: [level][slot][plane] plane values
   std::vector<double> dft_w_host;
 
+  // a run enqueued by gd_run_async that gd_sync (or the next call on the
+  // context) has not completed yet: its launch-event count and length
+  bool pending = false;
+  size_t pending_nev = 0;
+  size_t pending_nsteps = 0;
+  int pending_variant = 0;
+
   // TILED launches split in two: tiles with sources or monitored cells run
   // the full kernel on `side`, the others the lean kernel on `stream`
   hipStream_t side = nullptr;
@@ -125,6 +132,18 @@ thread_local std::string g_create_error;
 // some arguments, and the weights are part of the bit-exact contract.
 __attribute__((noinline)) double weight_cos(double a) { return std::cos(a); }
 __attribute__((noinline)) double weight_sin(double a) { return std::sin(a); }
+
+// Completes a run enqueued by gd_run_async: waits for the context's stream
+// and sets the run's timing (defined after gd_run). A no-op when none is
+// pending. Every call that changes the context or reads its results calls it
+// first, so the host buffers a run's uploads read are never refilled while
+// the run may still read them.
+int settle(gd_ctx* c);
+#define GD_SETTLE(c)                    \
+  do {                                  \
+    const int settle_rc_ = settle(c);   \
+    if (settle_rc_ != GD_OK) return settle_rc_; \
+  } while (0)
 
 int alloc_fields(gd_ctx* c, int copy) {
   for (int a = 0; a < 6; ++a) {
@@ -464,6 +483,7 @@ int gd_create(gd_ctx** out, size_t nx, size_t ny, size_t nz, float dx) {
 
 int gd_destroy(gd_ctx* c) {
   if (!c) return GD_OK;
+  if (c->pending) (void)settle(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int k = 0; k < 2; ++k)
     for (int a = 0; a < 6; ++a)
@@ -510,6 +530,7 @@ const char* gd_last_error(const gd_ctx* c) {
 
 int gd_set_storage(gd_ctx* c, int storage, const int* scale_exp) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (storage != GD_STORAGE_F32 && storage != GD_STORAGE_F64 && !is16(storage))
     return c->fail(GD_ERR_ARG, "unknown storage format");
   if (c->have_mat) return c->fail(GD_ERR_STATE, "set the storage format before the material");
@@ -549,6 +570,7 @@ int gd_get_storage(const gd_ctx* c, int* storage, int* scale_exp) {
 
 int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!coeff) return c->fail(GD_ERR_ARG, "null coefficient table");
   for (int a = 0; a < 12; ++a)
     if (!coeff[a]) return c->fail(GD_ERR_ARG, "null coefficient array");
@@ -579,6 +601,7 @@ int gd_set_coefficients(gd_ctx* c, const float* const coeff[12]) {
 
 int gd_set_material_format(gd_ctx* c, int format) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (format != GD_MATERIAL_AUTO && format != GD_MATERIAL_DENSE && format != GD_MATERIAL_INDEXED)
     return c->fail(GD_ERR_ARG, "unknown material format");
   if (c->have_mat) return c->fail(GD_ERR_STATE, "set the material format before the material");
@@ -600,6 +623,7 @@ int gd_get_material_format(const gd_ctx* c, int* format, int* ntable) {
 
 int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!cb) return c->fail(GD_ERR_ARG, "null cb array");
   // AUTO: the indexed form when Cb has at most GD_MAX_MATERIALS distinct
   // values (storage other than FP64), else the dense one. Both give the same
@@ -646,6 +670,7 @@ int gd_set_isotropic(gd_ctx* c, const float* cb, float db) {
 int gd_set_isotropic_indexed(gd_ctx* c, const uint8_t* id, const float* table, int ntable,
                              float db) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!id || !table) return c->fail(GD_ERR_ARG, "null id or table");
   if (ntable < 1 || ntable > GD_MAX_MATERIALS)
     return c->fail(GD_ERR_ARG, "ntable must be in 1..256");
@@ -667,6 +692,7 @@ int gd_set_isotropic_indexed(gd_ctx* c, const uint8_t* id, const float* table, i
 
 int gd_set_isotropic_eps(gd_ctx* c, const float* eps_r, float dt) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!eps_r) return c->fail(GD_ERR_ARG, "null eps array");
   gd::HostVec<float> cb(c->N);
   const double ddt = dt, ddx = c->dx;
@@ -689,6 +715,7 @@ int gd_set_isotropic_eps(gd_ctx* c, const float* eps_r, float dt) {
 int gd_add_source(gd_ctx* c, int comp, size_t i, size_t j, size_t k, const float* table,
                   size_t nsteps) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (comp < 0 || comp > 5) return c->fail(GD_ERR_ARG, "component must be 0..5");
   if (i >= c->nx || j >= c->ny || k >= c->nz) return c->fail(GD_ERR_ARG, "source outside grid");
   if (!table || nsteps == 0) return c->fail(GD_ERR_ARG, "empty source table");
@@ -712,6 +739,7 @@ int gd_add_source(gd_ctx* c, int comp, size_t i, size_t j, size_t k, const float
 
 int gd_clear_sources(gd_ctx* c) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   GD_HIP(c, hipStreamSynchronize(c->stream));
   for (auto& s : c->src) (void)hipFree(s.table);
   c->src.clear();
@@ -722,6 +750,7 @@ int gd_clear_sources(gd_ctx* c) {
 
 int gd_set_fields(gd_ctx* c, const float* const fields[6]) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!fields) return c->fail(GD_ERR_ARG, "null field table");
   for (int a = 0; a < 6; ++a)
     if (!fields[a]) return c->fail(GD_ERR_ARG, "null field array");
@@ -742,6 +771,7 @@ int gd_set_fields(gd_ctx* c, const float* const fields[6]) {
 
 int gd_get_field_f64(gd_ctx* c, int comp, double* out) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (comp < 0 || comp > 5 || !out) return c->fail(GD_ERR_ARG, "bad component or null output");
   GD_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<unsigned char> buf(c->N * c->esz);
@@ -752,6 +782,7 @@ int gd_get_field_f64(gd_ctx* c, int comp, double* out) {
 
 int gd_get_fields(gd_ctx* c, float* const fields[6]) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!fields) return c->fail(GD_ERR_ARG, "null field table");
   for (int a = 0; a < 6; ++a) {
     int rc = gd_get_field(c, a, fields[a]);
@@ -762,6 +793,7 @@ int gd_get_fields(gd_ctx* c, float* const fields[6]) {
 
 int gd_get_field(gd_ctx* c, int comp, float* out) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (comp < 0 || comp > 5 || !out) return c->fail(GD_ERR_ARG, "bad component or null output");
   GD_HIP(c, hipStreamSynchronize(c->stream));
   const size_t bytes = c->N * c->esz;
@@ -813,6 +845,7 @@ int gd_get_field(gd_ctx* c, int comp, float* out) {
 
 int gd_get_fields_f64(gd_ctx* c, double* const fields[6]) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!fields) return c->fail(GD_ERR_ARG, "null field table");
   for (int a = 0; a < 6; ++a) {
     int rc = gd_get_field_f64(c, a, fields[a]);
@@ -823,6 +856,7 @@ int gd_get_fields_f64(gd_ctx* c, double* const fields[6]) {
 
 int gd_get_plane_xy(gd_ctx* c, int comp, size_t k, float* out) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (comp < 0 || comp > 5 || !out || k >= c->nz)
     return c->fail(GD_ERR_ARG, "bad component, plane or null output");
   GD_HIP(c, hipStreamSynchronize(c->stream));
@@ -838,6 +872,7 @@ int gd_get_plane_xy(gd_ctx* c, int comp, size_t k, float* out) {
 
 int gd_reset(gd_ctx* c) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   for (int a = 0; a < 6; ++a)
     GD_HIP(c, hipMemsetAsync(c->f[c->cur][a], 0, c->N * c->esz, c->stream));
   const size_t dn = (size_t)c->dft_ncomp * c->dft_f.size() * c->dft_plane;
@@ -860,6 +895,7 @@ int gd_get_step(const gd_ctx* c, size_t* step) {
 
 int gd_dft_disable(gd_ctx* c) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   GD_HIP(c, hipStreamSynchronize(c->stream));
   if (c->dft_re) (void)hipFree(c->dft_re);
   if (c->dft_im) (void)hipFree(c->dft_im);
@@ -875,6 +911,7 @@ int gd_dft_disable(gd_ctx* c) {
 int gd_dft_config(gd_ctx* c, unsigned mask, int axis, size_t index, const double* f, size_t nf,
                   double scale) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (mask == 0 || mask > 63u) return c->fail(GD_ERR_ARG, "component mask must be in 1..63");
   if (axis < 0 || axis > 2) return c->fail(GD_ERR_ARG, "axis must be 0, 1 or 2");
   const size_t lim = axis == 0 ? c->nx : (axis == 1 ? c->ny : c->nz);
@@ -905,6 +942,7 @@ int gd_dft_config(gd_ctx* c, unsigned mask, int axis, size_t index, const double
 
 int gd_dft_get(gd_ctx* c, int comp, double* re, double* im) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!re || !im) return c->fail(GD_ERR_ARG, "null output");
   int slot = -1;
   for (int q = 0; q < c->dft_ncomp; ++q)
@@ -935,12 +973,14 @@ int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* 
 
 int gd_set_launch_timing(gd_ctx* c, int enable) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   c->launch_timing = enable != 0;
   return GD_OK;
 }
 
 int gd_last_run_stats(const gd_ctx* c, double* total_ms, double* kernel_ms, size_t* nlaunch) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(const_cast<gd_ctx*>(c));
   if (total_ms) *total_ms = c->last_total_ms;
   if (kernel_ms) *kernel_ms = c->last_kernel_ms;
   if (nlaunch) *nlaunch = c->last_nlaunch;
@@ -966,7 +1006,19 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
 }
 
 int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
+  const int rc = gd_run_async(c, variant, nsteps, tiling);
+  if (rc != GD_OK) return rc;
+  return settle(c);
+}
+
+int gd_sync(gd_ctx* c) {
   if (!c) return GD_ERR_ARG;
+  return settle(c);
+}
+
+int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
+  if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (variant != GD_VARIANT_PLAIN && variant != GD_VARIANT_TILED && variant != GD_VARIANT_FUSED)
     return c->fail(GD_ERR_ARG, "unknown variant");
   if (!c->have_mat) return c->fail(GD_ERR_STATE, "no material set");
@@ -1200,20 +1252,10 @@ int gd_run(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling) {
     }
   }
   GD_HIP(c, hipEventRecord(c->ev1, c->stream));
-  GD_HIP(c, hipStreamSynchronize(c->stream));
-  float ms = 0.0f;
-  GD_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  c->last_total_ms = ms;
-  if (c->launch_timing) {
-    double sum = 0.0;
-    for (size_t q = 0; q + 1 < nev; q += 2) {
-      float t = 0.0f;
-      GD_HIP(c, hipEventElapsedTime(&t, c->ev_pool[q], c->ev_pool[q + 1]));
-      sum += t;
-    }
-    c->last_kernel_ms = sum;
-    c->last_nlaunch = variant == GD_VARIANT_PLAIN ? nsteps * 2 : nev / 2;  // PLAIN: E and H
-  }
+  c->pending = true;
+  c->pending_nev = nev;
+  c->pending_nsteps = nsteps;
+  c->pending_variant = variant;
   c->step += nsteps;
   return GD_OK;
 }
@@ -1244,6 +1286,8 @@ int gd_calibration_copy(gd_ctx* c, size_t n, int reps, double* ms) {
 
 int gd_restrict_fields(gd_ctx* c, float* const out[6]) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
+  GD_SETTLE(c);
   if (!out) return c->fail(GD_ERR_ARG, "null output table");
   for (int a = 0; a < 6; ++a)
     if (!out[a]) return c->fail(GD_ERR_ARG, "null output array");
@@ -1266,6 +1310,7 @@ int gd_restrict_fields(gd_ctx* c, float* const out[6]) {
 int gd_run_corrected(gd_ctx* c, gd_nn* nn, size_t nsteps, int interval, const float cfg[6],
                      const gd_tiling* tiling) {
   if (!c) return GD_ERR_ARG;
+  GD_SETTLE(c);
   if (!nn || !cfg) return c->fail(GD_ERR_ARG, "null network or configuration");
   if (interval < 2) return c->fail(GD_ERR_ARG, "interval must be >= 2");
   if (c->storage != GD_STORAGE_F32 || !c->have_mat || c->mat_kind == gd::kMatFull)
@@ -1311,3 +1356,29 @@ int gd_run_corrected(gd_ctx* c, gd_nn* nn, size_t nsteps, int interval, const fl
 }
 
 }  // extern "C"
+
+namespace {
+
+int settle(gd_ctx* c) {
+  if (!c->pending) return GD_OK;
+  c->pending = false;
+  GD_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0.0f;
+  GD_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_total_ms = ms;
+  if (c->launch_timing) {
+    const size_t nev = c->pending_nev;
+    double sum = 0.0;
+    for (size_t q = 0; q + 1 < nev; q += 2) {
+      float t = 0.0f;
+      GD_HIP(c, hipEventElapsedTime(&t, c->ev_pool[q], c->ev_pool[q + 1]));
+      sum += t;
+    }
+    c->last_kernel_ms = sum;
+    // PLAIN: an E and an H launch per step
+    c->last_nlaunch = c->pending_variant == GD_VARIANT_PLAIN ? c->pending_nsteps * 2 : nev / 2;
+  }
+  return GD_OK;
+}
+
+}  // namespace

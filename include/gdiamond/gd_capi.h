@@ -220,6 +220,16 @@ int gd_dft_disable(gd_ctx* ctx);
  * cell or monitored cells on a second stream of the context, alongside the
  * others; both streams have the default priority. */
 int gd_run(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
+
+/* The same run, enqueued on the context's stream and returned from at once,
+ * so the caller's host work overlaps it (the reference's drivers always wait,
+ * cudaDeviceSynchronize before the copies back, gdiamond_gpu.cuh:914).
+ * gd_sync waits for it; so does the next call on the context that changes it
+ * or reads its results (fields, DFT sums, gd_last_run_stats), in stream order,
+ * so a result read after gd_run_async is the run's result. Argument errors
+ * are returned at once; errors of the device work surface at gd_sync. */
+int gd_run_async(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
+int gd_sync(gd_ctx* ctx);
 /* The tiling the planner would use for GD_VARIANT_TILED on this context. */
 int gd_plan_tiling(const gd_ctx* ctx, gd_tiling* out);
 /* The planner's tiling for a grid and storage format, without a context or
