@@ -87,7 +87,7 @@ typedef struct gd_tiling {
 /* Field storage formats (gd_set_storage). */
 enum {
   GD_STORAGE_F32 = 0, /* float fields, float arithmetic: the reference's format */
-  GD_STORAGE_F64 = 1, /* double fields and arithmetic (PLAIN only): reference runs */
+  GD_STORAGE_F64 = 1, /* double fields and arithmetic (PLAIN and FUSED): reference runs */
   GD_STORAGE_F16 = 2, /* binary16 of value*2^-k per component; float arithmetic */
   GD_STORAGE_BF16 = 3 /* bfloat16 of value*2^-k per component; float arithmetic */
 };
