@@ -8,16 +8,17 @@
 // per cell-update for FP32 fields with a dense Cb (49 with the material id),
 // against 76 for the in-place pair, which reads E and H twice.
 //
-// Geometry: a wave owns R consecutive y rows (here 1) of a 62-cell x segment
-// and a chunk of z planes; lanes 1..62 own x cells, lanes 0 and 63 are the x-1 and
+// Geometry: a wave owns R consecutive y rows of a 62-cell x segment and a
+// chunk of z planes; lanes 1..62 own x cells, lanes 0 and 63 are the x-1 and
 // x+1 halo, reached by DPP lane shifts. H of step n+1 at (i,j,k) needs E of
 // step n+1 at i+1, j+1 and k+1, so a wave also computes E on the row above
 // its rows (Ex and Ez, the components H's y-derivative reads) and on the
 // plane after its chunk, from the same inputs with the same operations as
 // the wave that owns those cells: no wave waits for another, and the kernel
-// has no LDS and no barriers. Each wave marches along z; H runs one plane
-// behind E. Many light waves per CU keep enough loads in flight to cover
-// the memory latency, as in the PLAIN kernels.
+// needs no LDS exchange and no barrier. Each wave marches along z; H runs
+// one plane behind E, and each plane's inputs are loaded one plane ahead.
+// Many short, light waves per CU keep enough loads in flight to cover the
+// memory latency, as in the PLAIN kernels.
 //
 // Physical boundary cells are copied unchanged, exactly as the reference
 // updates only 1..n-2 (gdiamond.hpp:1194-1221). The DFT of E and of H is
