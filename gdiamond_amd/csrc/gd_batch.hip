@@ -459,8 +459,8 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
   for (int i = 0; i < ninst; ++i) {
     const size_t nx = dims[3 * i], ny = dims[3 * i + 1], nz = dims[3 * i + 2];
     if (nx < 3 || ny < 3 || nz < 3 || nx > (1u << 20) || ny > (1u << 20) || nz > (1u << 20) ||
-        nx * ny * 4 >= (1ull << 32) || nz > (1ull << 36) / (nx * ny)) {
-      g_batch_create_error = "instance dims must be >= 3, a plane below 4 GiB and at most 2^36 cells";
+        nx * ny * 4 >= (1ull << 31) || nz > (1ull << 36) / (nx * ny)) {
+      g_batch_create_error = "instance dims must be >= 3, a plane below 2 GiB and at most 2^36 cells";
       return GD_ERR_ARG;
     }
   }

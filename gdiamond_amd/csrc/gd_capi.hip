@@ -1042,6 +1042,8 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
     zt = gd::ZTiling{t.blt, t.rows, t.waves, t.zchunk};
     if (c->storage == GD_STORAGE_F64)
       return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel runs FP32 and 16-bit storage only");
+    if (c->nx * c->ny * 4 >= (size_t(1) << 31))
+      return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel needs an xy plane below 512 Mi cells");
     if (!gd::ztiling_supported(zt, (gd::Storage)c->storage, c->mat_kind))
       return c->fail(GD_ERR_UNSUPPORTED, "unsupported tiling (blt 1..4, rows x waves in "
                                          "{2,4}x{4,8}, at least one owned row)");

@@ -391,6 +391,32 @@ __device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, 
   *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
 }
 
+// Store through a buffer resource spanning one plane (`plane_bytes` < 2^31
+// bytes from base + plane_off). A lane whose offset has bit 31 set
+// (kStoreDrop) lies past the resource's range, and the hardware drops its
+// store. Every lane of the wave then issues the same store instruction, so a
+// loop that stores only some cells on some planes still issues a fixed number
+// of vector-memory instructions per iteration. The compiler can then wait for
+// loads issued before those stores without also waiting for the stores
+// (the vector-memory counter completes in issue order).
+constexpr unsigned kStoreDrop = 0x80000000u;
+template <class S>
+__device__ __forceinline__ void stp_drop(void* base, size_t plane_off, unsigned plane_bytes,
+                                         unsigned off, S v) {
+  // dword 3 of the gfx9 buffer descriptor: 32-bit data format, no swizzle
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      static_cast<char*>(base) + plane_off, (short)0, (int)plane_bytes, 0x00020000);
+  if constexpr (sizeof(S) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)off, 0, 0);
+  else if constexpr (sizeof(S) == 2)
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), rs, (int)off, 0,
+                                          0);
+  else {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, (int)off, 0, 0);
+  }
+}
+
 }  // namespace gd
 
 struct gd_nn;

@@ -101,8 +101,8 @@ int gd_tiled_launch_noalloc(const gd_noalloc_launch* L) {
   if (!L) return arg_error("null launch description");
   const int nx = L->nx, ny = L->ny, nz = L->nz;
   if (nx < 3 || ny < 3 || nz < 3) return arg_error("grid must be at least 3 cells on every axis");
-  if ((unsigned long long)nx * (unsigned long long)ny * 4ull >= (1ull << 32))
-    return arg_error("an xy plane must stay below 4 GiB");
+  if ((unsigned long long)nx * (unsigned long long)ny * 4ull >= (1ull << 31))
+    return arg_error("an xy plane must stay below 2 GiB");
   if (L->nsteps < 1 || L->nsteps > GD_NOALLOC_MAX_STEPS)
     return arg_error("nsteps must be 1..GD_NOALLOC_MAX_STEPS (one launch)");
   if (L->n0 < 0 || (long long)L->n0 + L->nsteps > (long long)INT32_MAX)

@@ -137,6 +137,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   constexpr unsigned kLogES = ES == 4 ? 2 : (ES == 2 ? 1 : 0);
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
+  constexpr bool kDropStores = ES == 4;
   ZLds<B, W, kIdx>& lds = zlds<B, W, kIdx>();
   auto& lds_h = lds.h;
   auto& lds_e = lds.e;
@@ -171,6 +172,10 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
     }
   }
   const unsigned offm = (cgx + (unsigned)clampi(gy0 - 1, 0, ny - 1) * (unsigned)nx) * ES;
+  // store offsets: the cell's offset for owned cells, else past the plane
+  unsigned soff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) soff[r] = own[r] ? off[r] : kStoreDrop;
   // wave-uniform: does any lane of this wave hold a source column?
   const bool wave_src = kX && __builtin_amdgcn_ballot_w64(any_src) != 0;
 
@@ -273,6 +278,16 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
     }
   };
   prefetch(p_begin);
+  // With 4-byte fields the loop ends each position with 6R stores. Issuing
+  // 6R stores that the range check drops here as well makes the loop's entry
+  // look like its back edge: on both paths the prefetched plane is followed
+  // by 6R stores, so the loop's wait for that plane never includes the
+  // stores just issued.
+  if constexpr (kDropStores) {
+#pragma unroll
+    for (int q = 0; q < 6 * R; ++q)
+      stp_drop<S>(a.dst.c[q % 6], 0, (unsigned)plane_bytes, kStoreDrop, S{});
+  }
   __syncthreads();
 
   for (int p = p_begin; p <= p_end; ++p) {
@@ -425,8 +440,29 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
         lds_h[b - 1][p & 1][w][1][lane] = hn[R - 1][0];
       }
 
-      // ---- final level: owned cells go to the destination copy
-      if (b == B && own_any) {
+      // ---- final level: owned cells go to the destination copy. With
+      // 4-byte fields every position issues the same 6R stores, and lanes
+      // that own no cell of the plane are dropped by the buffer range check
+      // (kStoreDrop); 16-bit fields store only owned rows (measured faster
+      // for them: 281 against 253 Gcell/s at 512^3, B = 3).
+      if (b == B && kDropStores) {
+        const unsigned dropE = zE >= z_lo && zE < z_hi ? 0u : kStoreDrop;
+        const unsigned dropH = zH >= z_lo && zH < z_hi ? 0u : kStoreDrop;
+        const size_t oE = (size_t)clampi(zE, 0, nz - 1) * plane_bytes;
+        const size_t oH = (size_t)clampi(zH, 0, nz - 1) * plane_bytes;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            stp_drop<S>(a.dst.c[c], oE, (unsigned)plane_bytes, soff[r] | dropE,
+                        C::enc(en[r][c], sc.inv[c]));
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            stp_drop<S>(a.dst.c[3 + c], oH, (unsigned)plane_bytes, soff[r] | dropH,
+                        C::enc(hn[r][c], sc.inv[3 + c]));
+        }
+      }
+      if (b == B && !kDropStores && own_any) {
         if (zE >= z_lo && zE < z_hi) {
           const size_t oE = (size_t)zE * plane_bytes;
 #pragma unroll
