@@ -218,7 +218,9 @@ int gd_dft_disable(gd_ctx* ctx);
  * (tiling may be NULL). Synchronous: returns after the device finished. With
  * the planner's 4 x 8 tile a TILED launch runs the tiles that hold a source
  * cell or monitored cells on a second stream of the context, alongside the
- * others; both streams have the default priority. */
+ * others; both streams have the default priority. TILED and FUSED need an
+ * xy plane below 512 Mi cells (GD_ERR_UNSUPPORTED otherwise; PLAIN has no
+ * such limit). */
 int gd_run(gd_ctx* ctx, int variant, size_t nsteps, const gd_tiling* tiling);
 
 /* The same run, enqueued on the context's stream and returned from at once,
@@ -253,8 +255,9 @@ int gd_plan_tiling_for(size_t nx, size_t ny, size_t nz, int storage, gd_tiling* 
  * (32 MiB) that each GPU keeps until the process ends; every call on a GPU
  * shares that GPU's array, so calls that overlap on one GPU (other streams or
  * threads) race. The first call on a device looks up the array's address:
- * make it before capturing a graph. Sizes beyond the limits are rejected with
- * GD_ERR_ARG (text in gd_last_error(NULL)). fields_out receives every cell
+ * make it before capturing a graph. Sizes beyond the limits (among them an
+ * xy plane of 512 Mi cells or more) are rejected with GD_ERR_ARG (text in
+ * gd_last_error(NULL)). fields_out receives every cell
  * (boundary cells copied) and must not overlap fields_in; the DFT sums are
  * accumulated in place with the weights of absolute steps n0..n0+nsteps-1,
  * exactly as gd_run does, so a sequence of launches with n0 advancing by
@@ -336,7 +339,8 @@ enum {
                                (the unbatched way; for comparison) */
 };
 
-/* dims: 3*ninst sizes (nx, ny, nz per instance, each >= 3); every instance has
+/* dims: 3*ninst sizes (nx, ny, nz per instance, each >= 3, at most 2^20,
+ * an xy plane below 512 Mi cells, at most 2^36 cells); every instance has
  * cell size dx. Fields start at zero. */
 int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx);
 int gd_batch_destroy(gd_batch* b);
