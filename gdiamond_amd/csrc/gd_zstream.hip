@@ -127,7 +127,7 @@ __device__ __forceinline__ ZLds<B, W, kIdx>& zlds() {
 // and the DFT snapshots (tiles without a source cell in their window and
 // without owned monitored cells): their state no longer competes for
 // registers in the stream loop.
-template <int B, int R, int W, int kMat, class S, bool kX>
+template <int B, int R, int W, int kMat, class S, bool kX, bool kFixedStores = true>
 __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const int ty,
                                              const int tz) {
   using C = Codec<S>;
@@ -137,7 +137,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   constexpr unsigned kLogES = ES == 4 ? 2 : (ES == 2 ? 1 : 0);
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
-  constexpr bool kDropStores = ES == 4;
+  constexpr bool kDropStores = ES == 4 && kFixedStores;
   ZLds<B, W, kIdx>& lds = zlds<B, W, kIdx>();
   auto& lds_h = lds.h;
   auto& lds_e = lds.e;
@@ -441,10 +441,11 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
       }
 
       // ---- final level: owned cells go to the destination copy. With
-      // 4-byte fields every position issues the same 6R stores, and lanes
-      // that own no cell of the plane are dropped by the buffer range check
-      // (kStoreDrop); 16-bit fields store only owned rows (measured faster
-      // for them: 281 against 253 Gcell/s at 512^3, B = 3).
+      // 4-byte fields (single-problem launches) every position issues the
+      // same 6R stores, and lanes that own no cell of the plane are dropped
+      // by the buffer range check (kStoreDrop); 16-bit fields store only
+      // owned rows (measured faster for them: 281 against 253 Gcell/s at
+      // 512^3, B = 3), and so do the batch's tiles.
       if (b == B && kDropStores) {
         const unsigned dropE = zE >= z_lo && zE < z_hi ? 0u : kStoreDrop;
         const unsigned dropH = zH >= z_lo && zH < z_hi ? 0u : kStoreDrop;
@@ -586,10 +587,13 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_b
     a.n0 = n0;
     int tx, ty, tz;
     tile_coords(a, wi.tile, tx, ty, tz);
+    // branch-guarded stores: with the batch's small tiles and short streams
+    // the fixed-count form measured slower (256-instance batch 199-207
+    // against 213 Gcell/s)
     if (wi.full)
-      zstream_tile<B, R, W, kMat, S, true>(a, tx, ty, tz);
+      zstream_tile<B, R, W, kMat, S, true, false>(a, tx, ty, tz);
     else
-      zstream_tile<B, R, W, kMat, S, false>(a, tx, ty, tz);
+      zstream_tile<B, R, W, kMat, S, false, false>(a, tx, ty, tz);
   }
 }
 
