@@ -97,10 +97,17 @@ gd_tiling default_tiling(size_t nx, size_t ny, size_t nz) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Arrays of one instance start 16 KiB + 256 B past the 256-byte boundary
+// after the previous one, so that arrays a power of two in size (32^3, 64^3,
+// 128^3 FP32) do not put the same cell of every component on the same
+// memory channel and bank pattern.
+constexpr size_t kArrayStagger = 16640;
+size_t array_stride(size_t N) { return align_up(N * sizeof(float), 256) + kArrayStagger; }
+
 // Arena bytes of one instance: 12 field arrays and the dense Cb (FP32), the
 // material ids (1 byte per cell) and the id table.
 size_t inst_bytes(size_t N) {
-  return 13 * align_up(N * sizeof(float), 256) + align_up(N, 256) +
+  return 13 * array_stride(N) + align_up(N, 256) +
          align_up(gd::kMaxMaterials * sizeof(float), 256);
 }
 
@@ -514,7 +521,7 @@ int gd_batch_create(gd_batch** out, int ninst, const size_t* dims, float dx) {
   if (hipMemset(b->arena, 0, bytes) != hipSuccess) return bail("hipMemset(batch arena) failed");
   char* p = static_cast<char*>(b->arena);
   for (auto& in : b->inst) {
-    const size_t a = align_up(in.N * sizeof(float), 256);
+    const size_t a = array_stride(in.N);
     for (int k = 0; k < 2; ++k)
       for (int c = 0; c < 6; ++c, p += a) in.f[k][c] = reinterpret_cast<float*>(p);
     in.cb = reinterpret_cast<float*>(p);
