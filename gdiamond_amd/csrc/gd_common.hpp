@@ -379,15 +379,16 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // once before the stream loop; without the empty asm the compiler carries
 // them into the loop as 64-bit values and no longer sees that the high half
 // is zero, so every 4- or 8-byte access in the loop would cost a 64-bit add.
-// (For 2-byte elements the plain form measures faster.)
-template <class S>
+// (For 2-byte elements the plain form measures faster, and so it does for
+// every element size in the FUSED kernel, which passes kScalarBase = false.)
+template <class S, bool kScalarBase = (sizeof(S) >= 4)>
 __device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
-  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
+  if constexpr (kScalarBase) asm("" : "+v"(off));
   return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
 }
-template <class S>
+template <class S, bool kScalarBase = (sizeof(S) >= 4)>
 __device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
-  if constexpr (sizeof(S) >= 4) asm("" : "+v"(off));
+  if constexpr (kScalarBase) asm("" : "+v"(off));
   *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
 }
 

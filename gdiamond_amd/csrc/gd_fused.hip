@@ -130,10 +130,10 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
   const bool dft_h = wave_dft && (d.mask & 56u) != 0u;
 
   auto fld = [&](int c, size_t pz, unsigned q) {
-    return C::dec(ldp<S>(in.c[c], pz * ES, q * ES), sc.s[c]);
+    return C::dec(ldp<S, false>(in.c[c], pz * ES, q * ES), sc.s[c]);
   };
   auto coef = [&](int a, size_t pz, unsigned q) {
-    return ldp<T>(m.c[a], pz * sizeof(T), q * (unsigned)sizeof(T));
+    return ldp<T, false>(m.c[a], pz * sizeof(T), q * (unsigned)sizeof(T));
   };
 
   // H of step n on plane k-1 (own rows; Hy of the row above), and E of step
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
   using MT = typename std::conditional<kIdx, uint8_t, T>::type;
   S pe[R][3], ph[R][3], p_hzb, p_hxb, p_exa, p_eza, p_hxa, p_hya, p_hza;
   MT pm[R], pma;
-  auto raw = [&](int c, size_t pz, unsigned q) { return ldp<S>(in.c[c], pz * ES, q * ES); };
+  auto raw = [&](int c, size_t pz, unsigned q) { return ldp<S, false>(in.c[c], pz * ES, q * ES); };
   auto prefetch = [&](int kk) {
     const size_t pz = (size_t)kk * plane_cells;
 #pragma unroll
@@ -184,8 +184,8 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
     p_hza = raw(5, pz, cell_a);
     if constexpr (!kFull) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) pm[r] = ldp<MT>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell[r] * MS);
-      pma = ldp<MT>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell_a * MS);
+      for (int r = 0; r < R; ++r) pm[r] = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell[r] * MS);
+      pma = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell_a * MS);
     }
   };
   prefetch(k0);
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
         if (own[r]) {
 #pragma unroll
           for (int c = 0; c < 3; ++c)
-            stp<S>(out.c[c], pe_off, cell[r] * ES, C::enc(ep[r][c], sc.inv[c]));
+            stp<S, false>(out.c[c], pe_off, cell[r] * ES, C::enc(ep[r][c], sc.inv[c]));
         }
       if (k - 2 >= k0) {
         const size_t ph_off = (size_t)(k - 2) * plane_cells * ES;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
           if (own[r]) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              stp<S>(out.c[3 + c], ph_off, cell[r] * ES, C::enc(hs[r][c], sc.inv[3 + c]));
+              stp<S, false>(out.c[3 + c], ph_off, cell[r] * ES, C::enc(hs[r][c], sc.inv[3 + c]));
           }
       }
     }
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       if (own[r]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          stp<S>(out.c[c], pe_off, cell[r] * ES, C::enc(ep[r][c], sc.inv[c]));
+          stp<S, false>(out.c[c], pe_off, cell[r] * ES, C::enc(ep[r][c], sc.inv[c]));
       }
   }
   if (k_end - 1 >= k0) {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       if (own[r]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          stp<S>(out.c[3 + c], ph_off, cell[r] * ES, C::enc(hs[r][c], sc.inv[3 + c]));
+          stp<S, false>(out.c[3 + c], ph_off, cell[r] * ES, C::enc(hs[r][c], sc.inv[3 + c]));
       }
   }
 
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
       if (own[r]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          stp<S>(out.c[3 + c], ph * ES, cell[r] * ES, C::enc(hp[r][c], sc.inv[3 + c]));
+          stp<S, false>(out.c[3 + c], ph * ES, cell[r] * ES, C::enc(hp[r][c], sc.inv[3 + c]));
       }
       if (dft_h) {
         const long long p = own[r] ? dft_plane_index(d, g, i, j0 + r, kh) : -1;
