@@ -446,24 +446,25 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
       // by the buffer range check (kStoreDrop); 16-bit fields store only
       // owned rows (measured faster for them: 281 against 253 Gcell/s at
       // 512^3, B = 3), and so do the batch's tiles.
-      if (b == B && kDropStores) {
-        const unsigned dropE = zE >= z_lo && zE < z_hi ? 0u : kStoreDrop;
-        const unsigned dropH = zH >= z_lo && zH < z_hi ? 0u : kStoreDrop;
-        const size_t oE = (size_t)clampi(zE, 0, nz - 1) * plane_bytes;
-        const size_t oH = (size_t)clampi(zH, 0, nz - 1) * plane_bytes;
+      if constexpr (kDropStores) {
+        if (b == B) {
+          const unsigned dropE = zE >= z_lo && zE < z_hi ? 0u : kStoreDrop;
+          const unsigned dropH = zH >= z_lo && zH < z_hi ? 0u : kStoreDrop;
+          const size_t oE = (size_t)clampi(zE, 0, nz - 1) * plane_bytes;
+          const size_t oH = (size_t)clampi(zH, 0, nz - 1) * plane_bytes;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+          for (int r = 0; r < R; ++r) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            stp_drop<S>(a.dst.c[c], oE, (unsigned)plane_bytes, soff[r] | dropE,
-                        C::enc(en[r][c], sc.inv[c]));
+            for (int c = 0; c < 3; ++c)
+              stp_drop<S>(a.dst.c[c], oE, (unsigned)plane_bytes, soff[r] | dropE,
+                          C::enc(en[r][c], sc.inv[c]));
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            stp_drop<S>(a.dst.c[3 + c], oH, (unsigned)plane_bytes, soff[r] | dropH,
-                        C::enc(hn[r][c], sc.inv[3 + c]));
+            for (int c = 0; c < 3; ++c)
+              stp_drop<S>(a.dst.c[3 + c], oH, (unsigned)plane_bytes, soff[r] | dropH,
+                          C::enc(hn[r][c], sc.inv[3 + c]));
+          }
         }
-      }
-      if (b == B && !kDropStores && own_any) {
+      } else if (b == B && own_any) {
         if (zE >= z_lo && zE < z_hi) {
           const size_t oE = (size_t)zE * plane_bytes;
 #pragma unroll
@@ -825,8 +826,9 @@ hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const
   if (!ztiling_supported(t, st, mk)) return hipErrorInvalidValue;
   if (part.form == kZLean && !zstream_lean_supported(t)) return hipErrorInvalidValue;
   if (part.list && part.nlist < 1) return hipErrorInvalidValue;
-  // in-plane byte offsets are 32-bit
-  if ((unsigned long long)g.sxy * 4ull >= (1ull << 32)) return hipErrorInvalidValue;
+  // in-plane byte offsets are 32-bit, and a store offset with bit 31 set is
+  // dropped (stp_drop): every plane stays below 2^31 bytes
+  if ((unsigned long long)g.sxy * 4ull >= (1ull << 31)) return hipErrorInvalidValue;
   ZArgs a;
   a.src = src;
   a.dst = dst;
