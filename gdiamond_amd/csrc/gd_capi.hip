@@ -83,10 +83,13 @@ struct gd_ctx {
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   struct Split {
     int blt = 0;  // tiling the classification is for (0: none yet in this run)
-    std::vector<int> ids;  // tile ids: the extra tiles, then the lean ones
+    std::vector<int> ids;  // the extra tiles' ids, then the lean items
     size_t nextra = 0;
     int* d_ids = nullptr;
     size_t cap = 0;  // ids the device buffer holds
+    std::vector<gd::ZPack> packs;  // workgroups of remainder tiles (lean items -1-i)
+    gd::ZPack* d_packs = nullptr;
+    size_t pcap = 0;
   } split[2];
 
   bool launch_timing = false;
@@ -522,8 +525,10 @@ int gd_destroy(gd_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (int k = 0; k < 2; ++k)
     if (c->cal[k]) (void)hipFree(c->cal[k]);
-  for (auto& sp : c->split)
+  for (auto& sp : c->split) {
     if (sp.d_ids) (void)hipFree(sp.d_ids);
+    if (sp.d_packs) (void)hipFree(sp.d_packs);
+  }
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   if (c->join_ev) (void)hipEventDestroy(c->join_ev);
   for (int k = 0; k < 2; ++k) {
@@ -1193,7 +1198,9 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
     // With the planner's 4 x 8 tile a launch is split: the few tiles that hold
     // a source cell in their window or own monitored cells run the full
     // kernel on the side stream while every other tile runs the lean kernel
-    // (no source or DFT code in its loop) on the context's stream.
+    // (no source or DFT code in its loop) on the context's stream. In the
+    // lean kernel the tiles of a narrow remainder column or a short remainder
+    // row share workgroups (zstream_plan_items).
     const bool lean = gd::zstream_lean_supported(zt);
     const bool extras = s.n > 0 || d.ncomp > 0;
     if (lean && extras) {
@@ -1206,8 +1213,8 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
     // the context's stream (ahead of the launches that read them)
     std::vector<int> lean_ids;
     auto classify = [&](const gd::ZTiling& t, gd_ctx::Split& sp) -> int {
-      gd::zstream_classify(t, gd::zstream_scw(st), g, c->src_desc_host.data(), s.n, d, sp.ids,
-                           lean_ids);
+      gd::zstream_plan_items(t, gd::zstream_scw(st), g, c->src_desc_host.data(), s.n, d, sp.ids,
+                             lean_ids, sp.packs);
       sp.nextra = sp.ids.size();
       sp.ids.insert(sp.ids.end(), lean_ids.begin(), lean_ids.end());
       const size_t n = sp.ids.size();
@@ -1221,6 +1228,18 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
       }
       GD_HIP(c, hipMemcpyAsync(sp.d_ids, sp.ids.data(), n * sizeof(int), hipMemcpyHostToDevice,
                                c->stream));
+      const size_t np = sp.packs.size();
+      if (np > sp.pcap) {
+        if (sp.d_packs) (void)hipFree(sp.d_packs);
+        sp.d_packs = nullptr;
+        sp.pcap = 0;
+        hipError_t e = hipMalloc(&sp.d_packs, np * sizeof(gd::ZPack));
+        if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(tile packs)");
+        sp.pcap = np;
+      }
+      if (np)
+        GD_HIP(c, hipMemcpyAsync(sp.d_packs, sp.packs.data(), np * sizeof(gd::ZPack),
+                                 hipMemcpyHostToDevice, c->stream));
       sp.blt = t.blt;
       return GD_OK;
     };
@@ -1233,7 +1252,7 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
                                   c->mat_kind, g, s, d, (int)(c->step + r), (int)r, part, strm);
       };
       gd_ctx::Split* sp = nullptr;
-      if (lean && extras) {
+      if (lean) {
         sp = &c->split[t.blt == zt.blt ? 0 : 1];
         if (sp->blt != t.blt) {
           const int rc = classify(t, *sp);
@@ -1249,15 +1268,15 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
       }
       const int ne = sp ? (int)sp->nextra : 0;
       const int nl = sp ? (int)(sp->ids.size() - sp->nextra) : 0;
-      if (!lean || (sp && nl == 0)) {
+      if (!sp || nl == 0) {
         GD_HIP(c, launch(gd::ZPart{gd::kZFull, nullptr, 0}, c->stream));
-      } else if (!sp || ne == 0) {
-        GD_HIP(c, launch(gd::ZPart{gd::kZLean, nullptr, 0}, c->stream));
+      } else if (ne == 0) {
+        GD_HIP(c, launch(gd::ZPart{gd::kZLean, sp->d_ids, nl, sp->d_packs}, c->stream));
       } else {
         GD_HIP(c, hipEventRecord(c->fork_ev, c->stream));
         GD_HIP(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
         GD_HIP(c, launch(gd::ZPart{gd::kZFull, sp->d_ids, ne}, c->side));
-        GD_HIP(c, launch(gd::ZPart{gd::kZLean, sp->d_ids + ne, nl}, c->stream));
+        GD_HIP(c, launch(gd::ZPart{gd::kZLean, sp->d_ids + ne, nl, sp->d_packs}, c->stream));
         GD_HIP(c, hipEventRecord(c->join_ev, c->side));
         GD_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
       }

@@ -491,10 +491,24 @@ bool ztiling_supported(const ZTiling& t, Storage st, MatKind mk);
 // tiles; null runs every tile. The two lists of one classification
 // (zstream_classify) together cover every tile once.
 enum ZForm { kZFull = 0, kZLean = 1 };
+// One workgroup of the lean form that runs several smaller tiles of one z
+// chunk side by side: lane groups of g lanes (64, 32 or 16) times h wave
+// groups (1 or 2) of W/h waves; sub-tile (q, k) is tile (tx[q][k], ty[q][k])
+// and owns cells when bit 2q+k of `own` is set. A lean list entry -1-i names
+// packs[i].
+struct ZPack {
+  int tz;
+  int g;
+  int h;
+  int own;
+  int tx[4][2];
+  int ty[4][2];
+};
 struct ZPart {
   int form;
   const int* list;
   int nlist;
+  const ZPack* packs = nullptr;
 };
 // The lean form exists for the 4 x 8 tile (the planner's).
 bool zstream_lean_supported(const ZTiling& t);
@@ -507,6 +521,14 @@ int zstream_scw(Storage st);
 // the tile order of column width scw.
 void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src, int nsrc,
                       const Dft& d, std::vector<int>& extra, std::vector<int>& lean);
+// Host: the work items of one single-problem launch pair. `extra`: ids of the
+// tiles that need the source or DFT code (full form). `lean`: tile ids, or
+// -1-i for packs[i], which gathers tiles of the remainder column or row
+// (when the tile size leaves a narrow or short one) into shared workgroups.
+// Both lists follow the kernels' tile order.
+void zstream_plan_items(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src,
+                        int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean,
+                        std::vector<ZPack>& packs);
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,
                           const Sources& s, const Dft& d, int n0, int run_step0,

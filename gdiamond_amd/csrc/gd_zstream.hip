@@ -127,9 +127,19 @@ __device__ __forceinline__ ZLds<B, W, kIdx>& zlds() {
 // and the DFT snapshots (tiles without a source cell in their window and
 // without owned monitored cells): their state no longer competes for
 // registers in the stream loop.
+//
+// Sub-tiles: a workgroup may also run several smaller tiles side by side
+// (the remainder column or row of a grid that the tile size does not divide):
+// lanes form groups of G (64, 32 or 16) lanes and waves groups of WH (W or
+// W/2) waves, each group pair its own tile (tx, ty), owning the cells at
+// least B lanes and B rows inside its group. Lane shifts and the exchanged
+// rows cross group edges and bring another tile's values there, but those
+// reach only cells outside every owned cone, as the grid clamps do. A lane
+// with `active` false owns nothing (padding of a partly filled workgroup).
 template <int B, int R, int W, int kMat, class S, bool kX, bool kFixedStores = true>
 __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const int ty,
-                                             const int tz) {
+                                             const int tz, const int G = 64, const int WH = W,
+                                             const bool active = true) {
   using C = Codec<S>;
   constexpr bool kFull = kMat == kMatFull;
   constexpr bool kIdx = kMat == kMatIdx;
@@ -151,10 +161,12 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   const Scales& sc = a.sc;
   const float dx = a.g.dx;
 
-  const int gx = tx * OX - B + lane;
-  const int gy0 = ty * OY - B + w * R;
+  const int ll = lane & (G - 1);  // lane within its sub-tile (G: 64, 32 or 16)
+  const int wl = w & (WH - 1);    // wave within its sub-tile (WH: W or W/2)
+  const int gx = tx * OX - B + ll;
+  const int gy0 = ty * OY - B + wl * R;
   const bool intx = gx >= 1 && gx <= nx - 2;
-  const bool ownx = lane >= B && lane < B + OX && gx < nx;
+  const bool ownx = active && ll >= B && ll < G - B && gx < nx;
   const unsigned cgx = (unsigned)clampi(gx, 0, nx - 1);
   bool upd_xy[R], own[R];
   unsigned off[R];   // in-plane byte offset, clamped into the grid
@@ -163,8 +175,8 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   for (int r = 0; r < R; ++r) {
     const int gy = gy0 + r;
     upd_xy[r] = intx && gy >= 1 && gy <= ny - 2;
-    const int ly = w * R + r;
-    own[r] = ownx && ly >= B && ly < B + OY && gy < ny;
+    const int ly = wl * R + r;
+    own[r] = ownx && ly >= B && ly < WH * R - B && gy < ny;
     off[r] = (cgx + (unsigned)clampi(gy, 0, ny - 1) * (unsigned)nx) * ES;
     for (int q = 0; q < a.s.n; ++q) {
       const SourceDesc sd = a.s.d[q];
@@ -512,6 +524,17 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   }
 }
 
+// Copies a wave-uniform object from global memory into registers through
+// readfirstlane (a scalar value for every word the caller uses).
+template <class T>
+__device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized objects only");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+#pragma unroll
+  for (unsigned q = 0; q < sizeof(T) / 4; ++q) d[q] = __builtin_amdgcn_readfirstlane(s[q]);
+}
+
 // Tile of this workgroup: launch position xcd_tile(blockIdx) in tile order,
 // or, with a list, the tile whose id (tx + ntx*(ty + nty*tz)) the list holds
 // at that position. Lists are in tile order, so each XCD still takes a
@@ -540,24 +563,42 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_k
 }
 
 // The same without the source and DFT code, for tiles that need neither.
+// A negative list entry -1-q names packed workgroup q (sub-tiles, see
+// zstream_tile).
 template <int B, int R, int W, int kMat, class S>
 __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_lean_kernel(
-    ZArgs a, const int* __restrict__ list) {
-  int tx, ty, tz;
-  launch_tile(a, list, tx, ty, tz);
-  zstream_tile<B, R, W, kMat, S, false>(a, tx, ty, tz);
+    ZArgs a, const int* __restrict__ list, const ZPack* __restrict__ packs) {
+  const int pos = xcd_tile(blockIdx.x, gridDim.x);
+  const int item = list ? list[pos] : 0;
+  int tx, ty, tz, G = 64, WH = W;
+  bool active = true;
+  if (list && item < 0) {
+    ZPack pk;
+    uniform_copy(&pk, packs + (-1 - item));
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    G = pk.g;
+    WH = W / pk.h;
+    const int gi = lane / G, hi = w / WH;
+    tx = pk.tx[0][0];
+    ty = pk.ty[0][0];
+    active = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (gi == q && hi == h) {
+          tx = pk.tx[q][h];
+          ty = pk.ty[q][h];
+          active = (pk.own >> (2 * q + h)) & 1;
+        }
+    tz = pk.tz;
+  } else {
+    launch_tile(a, list, tx, ty, tz);
+  }
+  zstream_tile<B, R, W, kMat, S, false>(a, tx, ty, tz, G, WH, active);
 }
 
-// Copies a wave-uniform object from global memory into registers through
-// readfirstlane (a scalar value for every word the caller uses).
-template <class T>
-__device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
-  static_assert(sizeof(T) % 4 == 0, "word-sized objects only");
-  const int* s = reinterpret_cast<const int*>(src);
-  int* d = reinterpret_cast<int*>(dst);
-#pragma unroll
-  for (unsigned q = 0; q < sizeof(T) / 4; ++q) d[q] = __builtin_amdgcn_readfirstlane(s[q]);
-}
 
 // Persistent batch kernel: workgroups take (instance, tile) items from the
 // launch's queue until it runs dry (every wave of a workgroup sees the same
@@ -673,7 +714,7 @@ void launch_k(const ZArgs& a, const ZPart& part, hipStream_t stream) {
     // the lean form is built for the planner's 4 x 8 tile only
     if constexpr (R == 4 && W == 8)
       hipLaunchKernelGGL((zstream_lean_kernel<B, R, W, kMat, S>), dim3(n), dim3(64 * W), 0,
-                         stream, a, part.list);
+                         stream, a, part.list, part.packs);
   } else {
     hipLaunchKernelGGL((zstream_kernel<B, R, W, kMat, S>), dim3(n), dim3(64 * W), 0, stream, a,
                        part.list);
@@ -782,13 +823,36 @@ bool zstream_lean_supported(const ZTiling& t) { return t.rows == 4 && t.waves ==
 // Column width of the tile order (ZArgs::scw) for a storage format.
 int zstream_scw(Storage st) { return st == kF32 ? 10 : 0; }
 
+namespace {
+// Does tile (tx, ty, tz) need the source or DFT code: a source cell in its
+// input window (x [x0, x0+64), y [y0, y0+rows*waves)) and in the planes its
+// stream computes (z_lo-2B .. z_hi-1+B), or owned cells on the monitored
+// plane?
+bool tile_needs_full(const ZTiling& t, const Grid& g, const SourceDesc* src, int nsrc,
+                     const Dft& d, int tx, int ty, int tz) {
+  const int B = t.blt;
+  const int ox = 64 - 2 * B, oy = t.rows * t.waves - 2 * B;
+  const int x0 = tx * ox - B, y0 = ty * oy - B;
+  const int z_lo = tz * t.zchunk, z_hi = std::min(g.nz, z_lo + t.zchunk);
+  for (int q = 0; q < nsrc; ++q)
+    if (src[q].i >= x0 && src[q].i < x0 + 64 && src[q].j >= y0 &&
+        src[q].j < y0 + t.rows * t.waves && src[q].k >= z_lo - 2 * B &&
+        src[q].k <= z_hi - 1 + B)
+      return true;
+  if (d.ncomp > 0) {
+    if (d.axis == 2) return d.index >= z_lo && d.index < z_hi;
+    if (d.axis == 0) return d.index >= tx * ox && d.index < tx * ox + ox;
+    return d.index >= ty * oy && d.index < ty * oy + oy;
+  }
+  return false;
+}
+}  // namespace
+
 void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src, int nsrc,
                       const Dft& d, std::vector<int>& extra, std::vector<int>& lean) {
   ZArgs a{};
   ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
   a.scw = scw;
-  const int B = t.blt;
-  const int ox = 64 - 2 * B, oy = t.rows * t.waves - 2 * B;
   extra.clear();
   lean.clear();
   const int ntiles = a.ntx * a.nty * a.ntz;
@@ -797,25 +861,81 @@ void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc
     // together
     int tx, ty, tz;
     tile_coords(a, pos, tx, ty, tz);
-    // the tile's input window: x [x0, x0+64), y [y0, y0+rows*waves); its
-    // stream computes planes z_lo-2B .. z_hi-1+B
-    const int x0 = tx * ox - B, y0 = ty * oy - B;
-    const int z_lo = tz * t.zchunk, z_hi = std::min(g.nz, z_lo + t.zchunk);
-    bool x = false;
-    for (int q = 0; q < nsrc && !x; ++q)
-      x = src[q].i >= x0 && src[q].i < x0 + 64 && src[q].j >= y0 &&
-          src[q].j < y0 + t.rows * t.waves && src[q].k >= z_lo - 2 * B &&
-          src[q].k <= z_hi - 1 + B;
-    // owned cells on the monitored plane
-    if (!x && d.ncomp > 0) {
-      if (d.axis == 2)
-        x = d.index >= z_lo && d.index < z_hi;
-      else if (d.axis == 0)
-        x = d.index >= tx * ox && d.index < tx * ox + ox;
-      else
-        x = d.index >= ty * oy && d.index < ty * oy + oy;
+    (tile_needs_full(t, g, src, nsrc, d, tx, ty, tz) ? extra : lean)
+        .push_back(tx + a.ntx * (ty + a.nty * tz));
+  }
+}
+
+void zstream_plan_items(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src,
+                        int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean,
+                        std::vector<ZPack>& packs) {
+  packs.clear();
+  ZArgs a{};
+  ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
+  const int B = t.blt, W = t.waves, R = t.rows;
+  const int ox = 64 - 2 * B, oy = R * W - 2 * B;
+  // remainder column and row: cells of the last tile column / row
+  const int rx = g.nx - (a.ntx - 1) * ox, ry = g.ny - (a.nty - 1) * oy;
+  int G = 64, H = 1;
+  if (scw <= 0 || a.ntx <= scw) {  // plain x-fastest order
+    if (a.ntx > 1 && rx + 2 * B <= 16)
+      G = 16;
+    else if (a.ntx > 1 && rx + 2 * B <= 32)
+      G = 32;
+    if (a.nty > 1 && W % 2 == 0 && ry + 2 * B <= (W / 2) * R) H = 2;
+  }
+  if (G == 64 && H == 1) {
+    zstream_classify(t, scw, g, src, nsrc, d, extra, lean);
+    return;
+  }
+  extra.clear();
+  lean.clear();
+  // pending sub-tiles of the remainder column (x packs: 64/G per workgroup,
+  // full height) and row (y packs: 2 per workgroup, half height)
+  std::vector<std::pair<int, int>> xp, yp;
+  auto emit = [&](std::vector<std::pair<int, int>>& pend, int tz, bool xpack) {
+    if (pend.empty()) return;
+    ZPack pk{};
+    pk.tz = tz;
+    pk.g = xpack ? G : 64;
+    pk.h = xpack ? 1 : 2;
+    pk.own = 0;
+    const int slots = xpack ? 64 / G : 2;
+    for (int q = 0; q < 4; ++q)
+      for (int k = 0; k < 2; ++k) {
+        pk.tx[q][k] = pend[0].first;
+        pk.ty[q][k] = pend[0].second;
+      }
+    for (int i = 0; i < slots && i < (int)pend.size(); ++i) {
+      const int q = xpack ? i : 0, k = xpack ? 0 : i;
+      pk.tx[q][k] = pend[i].first;
+      pk.ty[q][k] = pend[i].second;
+      pk.own |= 1 << (2 * q + k);
     }
-    (x ? extra : lean).push_back(tx + a.ntx * (ty + a.nty * tz));
+    lean.push_back(-1 - (int)packs.size());
+    packs.push_back(pk);
+    pend.clear();
+  };
+  for (int tz = 0; tz < a.ntz; ++tz) {
+    for (int ty = 0; ty < a.nty; ++ty)
+      for (int tx = 0; tx < a.ntx; ++tx) {
+        const int id = tx + a.ntx * (ty + a.nty * tz);
+        if (tile_needs_full(t, g, src, nsrc, d, tx, ty, tz)) {
+          extra.push_back(id);
+        } else if (G < 64 && tx == a.ntx - 1) {
+          // narrow tile: its G lanes cover the rx remainder cells plus halos
+          xp.emplace_back(tx, ty);
+          if ((int)xp.size() == 64 / G) emit(xp, tz, true);
+        } else if (H == 2 && ty == a.nty - 1) {
+          // short tile: W/2 waves cover the ry remainder rows plus halos
+          yp.emplace_back(tx, ty);
+          if ((int)yp.size() == 2) emit(yp, tz, false);
+        } else {
+          lean.push_back(id);
+        }
+      }
+    emit(xp, tz, true);
+    emit(yp, tz, false);
   }
 }
 

@@ -359,3 +359,38 @@ def test_config_spheres_512_full_run_vs_oracle(gpu, oracle):
             assert bits_equal(re, d["re"][s]), (variant, c, describe_diff(re, d["re"][s]))
             assert bits_equal(im, d["im"][s]), (variant, c, describe_diff(im, d["im"][s]))
     assert np.abs(d["re"]).max() > 0
+
+
+@pytest.mark.parametrize("shape,blt", [((120, 80, 40), 4), ((120, 80, 40), 3), ((76, 53, 30), 4),
+                                       ((200, 150, 37), 2), ((66, 34, 21), 1),
+                                       ((57, 25, 9), 4)])
+def test_remainder_tile_packs(gpu, oracle, shape, blt):
+    """The lean form runs the tiles of a narrow remainder column (16 or 32
+    lanes per tile, four or two per workgroup) and of a short remainder row
+    (half-height tiles, two per workgroup) side by side; a partly filled
+    workgroup pads with lanes that own nothing. Remainder tiles that hold a
+    source or monitored cells stay whole tiles of the full form. 120 = 2*56+8
+    cells at B=4 (16-lane tiles), 76 = 56+20 (32-lane tiles), 80 = 3*24+8 rows
+    (half-height tiles); ten steps: launches of B levels and a shorter last
+    one, each planned anew."""
+    T = 10
+    eps, cb, db = iso_problem(gpu, shape, seed=13, nspheres=8, rmin=2, rmax=6)
+    nx, ny, nz = shape
+    src = gpu.gaussian_pulse_table(T, 3.0, 2.0, 0.06)
+    setup = dict(cb=cb, db=db, fields=random_fields(shape, 23, scale=0.05),
+                 sources=[(gpu.EZ, *centre(shape), src), (gpu.HY, nx - 3, ny // 3, nz // 2, src),
+                          (gpu.EX, nx // 3, ny - 2, 2, src * 0.5)],
+                 dft=dict(comps=[2, 3], axis=0, index=nx - 2, f_cps=np.array([0.05, 0.08])))
+    want, (wre, wim) = oracle_run(oracle, shape, setup, T)
+    t = dict(blt=blt, rows=4, waves=8, zchunk=8)
+    say(f"[packs] {shape} {t}")
+    got, dft = run_variant(gpu, shape, setup, T, gpu.TILED, t)
+    assert_fields_equal(got, want, f"packs {shape} {t}")
+    for n, c in enumerate(setup["dft"]["comps"]):
+        assert bits_equal(dft[c][0], wre[n]) and bits_equal(dft[c][1], wim[n]), \
+            f"packs {shape} {t} DFT comp {c}"
+    # without sources and DFT every remainder tile is packed
+    setup2 = dict(cb=cb, db=db, fields=setup["fields"])
+    want2, _ = oracle_run(oracle, shape, setup2, T)
+    got2, _ = run_variant(gpu, shape, setup2, T, gpu.TILED, t)
+    assert_fields_equal(got2, want2, f"packs (lean only) {shape} {t}")
