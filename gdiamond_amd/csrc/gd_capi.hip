@@ -1243,7 +1243,11 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
       sp.blt = t.blt;
       return GD_OK;
     };
+    // the snapshot buffer holds kMaxBlt levels: the DFT fold runs once per
+    // pair of launches (B = 3, 4), which saves a kernel and a dependency
     size_t r = 0;
+    int snap_levels = 0;  // levels in the snapshot buffer not folded yet
+    size_t snap_r0 = 0;   // run step of the first of them
     while (r < nsteps) {
       gd::ZTiling t = zt;
       t.blt = (int)std::min<size_t>((size_t)zt.blt, nsteps - r);
@@ -1266,6 +1270,7 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
         if (!a || !b) return c->fail(GD_ERR_HIP, "event creation failed");
         GD_HIP(c, hipEventRecord(a, c->stream));
       }
+      d.snap_base = snap_levels;
       const int ne = sp ? (int)sp->nextra : 0;
       const int nl = sp ? (int)(sp->ids.size() - sp->nextra) : 0;
       if (!sp || nl == 0) {
@@ -1281,7 +1286,16 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
         GD_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
       }
       if (c->launch_timing) GD_HIP(c, hipEventRecord(b, c->stream));
-      if (d.ncomp) GD_HIP(c, gd::launch_dft_snapshot(d, t.blt, (int)r, c->stream));
+      if (d.ncomp) {
+        snap_levels += t.blt;
+        const size_t rn = r + (size_t)t.blt;
+        const size_t next = std::min<size_t>((size_t)zt.blt, nsteps - rn);
+        if (rn >= nsteps || snap_levels + (int)next > gd::kMaxBlt) {
+          GD_HIP(c, gd::launch_dft_snapshot(d, snap_levels, (int)snap_r0, c->stream));
+          snap_levels = 0;
+          snap_r0 = rn;
+        }
+      }
       c->cur = 1 - c->cur;
       r += (size_t)t.blt;
     }

@@ -270,6 +270,7 @@ struct Dft {
   double* re;        // [slot][f][plane]
   double* im;
   float* snap;       // TILED: plane values per level, [level][slot][plane]
+  int snap_base;     // TILED: level slot of the launch's first level
 };
 
 struct Grid {

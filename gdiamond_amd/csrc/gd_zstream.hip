@@ -429,14 +429,16 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
         for (int c = 0; c < 3; ++c) {
           if (okE && ((a.d.mask >> c) & 1u)) {
             const int slot = __builtin_popcount(a.d.mask & ((1u << c) - 1u));
-            float* sp = a.d.snap + ((size_t)(b - 1) * a.d.ncomp + slot) * (size_t)a.d.plane;
+            float* sp = a.d.snap +
+                        ((size_t)(b - 1 + a.d.snap_base) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
               if (dft_row(r)) sp[dft_base(r) + zE * dft_zmul] = en[r][c];
           }
           if (okH && ((a.d.mask >> (3 + c)) & 1u)) {
             const int slot = __builtin_popcount(a.d.mask & ((1u << (3 + c)) - 1u));
-            float* sp = a.d.snap + ((size_t)(b - 1) * a.d.ncomp + slot) * (size_t)a.d.plane;
+            float* sp = a.d.snap +
+                        ((size_t)(b - 1 + a.d.snap_base) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
               if (dft_row(r)) sp[dft_base(r) + zH * dft_zmul] = hn[r][c];
