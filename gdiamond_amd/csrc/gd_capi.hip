@@ -1213,7 +1213,7 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
     // the context's stream (ahead of the launches that read them)
     std::vector<int> lean_ids;
     auto classify = [&](const gd::ZTiling& t, gd_ctx::Split& sp) -> int {
-      gd::zstream_plan_items(t, gd::zstream_scw(st), g, c->src_desc_host.data(), s.n, d, sp.ids,
+      gd::zstream_plan_items(t, st, g, c->src_desc_host.data(), s.n, d, sp.ids,
                              lean_ids, sp.packs);
       sp.nextra = sp.ids.size();
       sp.ids.insert(sp.ids.end(), lean_ids.begin(), lean_ids.end());

@@ -523,13 +523,30 @@ int zstream_scw(Storage st);
 void zstream_classify(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src, int nsrc,
                       const Dft& d, std::vector<int>& extra, std::vector<int>& lean);
 // Host: the work items of one single-problem launch pair. `extra`: ids of the
-// tiles that need the source or DFT code (full form). `lean`: tile ids, or
-// -1-i for packs[i], which gathers tiles of the remainder column or row
-// (when the tile size leaves a narrow or short one) into shared workgroups.
-// Both lists follow the kernels' tile order.
-void zstream_plan_items(const ZTiling& t, int scw, const Grid& g, const SourceDesc* src,
+// tiles that need the source or DFT code (full form), in the kernels' tile
+// order. `lean`: tile ids, or -1-i for packs[i], which gathers tiles of the
+// remainder column or row (when the tile size leaves a narrow or short one)
+// into shared workgroups, in chunk-group order (zstream_chunk_groups) for
+// FP32 fields and in the tile order of zstream_scw(st) otherwise.
+void zstream_plan_items(const ZTiling& t, Storage st, const Grid& g, const SourceDesc* src,
                         int nsrc, const Dft& d, std::vector<int>& extra, std::vector<int>& lean,
                         std::vector<ZPack>& packs);
+// Host: reorders a lean list (items of ntz z chunks of ncol x-y columns each,
+// chunk by chunk in the tile order) into groups of gc consecutive columns,
+// each group walked through every z chunk before the next group. Each XCD
+// takes a contiguous run of the list and holds 32 workgroups at a time, so a
+// chunk starts gc items, i.e. gc/32 of a chunk's stream, after the chunk
+// below it in the same columns; with gc = 32 D / (D + 2B) (D planes per
+// chunk, B levels) that is when the chunk below reaches the 2B planes the two
+// share (its top halo, the other's bottom halo), so both read those planes
+// at about the same time. (FP32, 32-plane chunks: 512^3 227 -> 236
+// Gcell/s, 768^3 236 -> 244, 1024^3 233 -> 243 and 234 -> 253 on two
+// boxes. The L2 fetch count per launch does not drop, 6.4 GB before and
+// after at 512^3, so the second of the two simultaneous reads is served
+// below L2; the timing matters: 20 or 32 columns per group instead of 27 at
+// 48-plane chunks ran 4-5 % slower than the plain order.)
+void zstream_chunk_groups(const ZTiling& t, int ntz, int ncol, std::vector<int>& lean,
+                          const std::vector<ZPack>& packs);
 hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const Fields& dst,
                           const Scales& sc, const Material& m, MatKind mk, const Grid& g,
                           const Sources& s, const Dft& d, int n0, int run_step0,
