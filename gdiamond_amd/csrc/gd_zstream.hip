@@ -271,6 +271,20 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   MT pcb[R];
   auto prefetch = [&](int pp) {
     const size_t pz = (size_t)min(pp, nz - 1) * plane_bytes;
+    // The material ids (or Cb) first: the compiler moves their registers at
+    // the loop's back edge, which needs those loads complete, and the
+    // vector-memory counter completes in issue order, so the field loads
+    // issued after them (and this position's stores) stay in flight there.
+    // Issued last, they made that wait a wait for every outstanding access
+    // (binary16: vmcnt(0) at the back edge): 512^3 binary16 286.7 -> 298.6,
+    // 1024^3 314.2 -> 321.9, FP32 236.9 -> 241.0 Gcell/s.
+    if constexpr (kIdx) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) pcb[r] = ldp<uint8_t>(a.m.id, pz >> kLogES, off[r] >> kLogES);
+    } else if constexpr (!kFull) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) pcb[r] = ldp<float>(a.m.cb, pz * kCbMul, off[r] * kCbMul);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
@@ -281,13 +295,6 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
     }
     phmz = ldp<S>(a.src.c[5], pz, offm);
     phmx = ldp<S>(a.src.c[3], pz, offm);
-    if constexpr (kIdx) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) pcb[r] = ldp<uint8_t>(a.m.id, pz >> kLogES, off[r] >> kLogES);
-    } else if constexpr (!kFull) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) pcb[r] = ldp<float>(a.m.cb, pz * kCbMul, off[r] * kCbMul);
-    }
   };
   prefetch(p_begin);
   // With 4-byte fields the loop ends each position with 6R stores. Issuing
