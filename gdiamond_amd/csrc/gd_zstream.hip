@@ -136,7 +136,12 @@ __device__ __forceinline__ ZLds<B, W, kIdx>& zlds() {
 // rows cross group edges and bring another tile's values there, but those
 // reach only cells outside every owned cone, as the grid clamps do. A lane
 // with `active` false owns nothing (padding of a partly filled workgroup).
-template <int B, int R, int W, int kMat, class S, bool kX, bool kFixedStores = true>
+//
+// kSingle: the tile of a single-problem launch (fixed-count stores for
+// 4-byte fields, material loaded ahead of the fields) or, false, of the
+// batch kernel (branch-guarded stores, material loaded after the fields),
+// the faster form for each.
+template <int B, int R, int W, int kMat, class S, bool kX, bool kSingle = true>
 __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const int ty,
                                              const int tz, const int G = 64, const int WH = W,
                                              const bool active = true) {
@@ -147,7 +152,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   constexpr unsigned kLogES = ES == 4 ? 2 : (ES == 2 ? 1 : 0);
   constexpr int OX = 64 - 2 * B;
   constexpr int OY = W * R - 2 * B;
-  constexpr bool kDropStores = ES == 4 && kFixedStores;
+  constexpr bool kDropStores = ES == 4 && kSingle;
   ZLds<B, W, kIdx>& lds = zlds<B, W, kIdx>();
   auto& lds_h = lds.h;
   auto& lds_e = lds.e;
@@ -269,15 +274,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   constexpr unsigned kCbMul = 4 / ES;  // FP32 Cb offsets from field offsets
   S pe[R][3], ph[R][3], phmz, phmx;
   MT pcb[R];
-  auto prefetch = [&](int pp) {
-    const size_t pz = (size_t)min(pp, nz - 1) * plane_bytes;
-    // The material ids (or Cb) first: the compiler moves their registers at
-    // the loop's back edge, which needs those loads complete, and the
-    // vector-memory counter completes in issue order, so the field loads
-    // issued after them (and this position's stores) stay in flight there.
-    // Issued last, they made that wait a wait for every outstanding access
-    // (binary16: vmcnt(0) at the back edge): 512^3 binary16 286.7 -> 298.6,
-    // 1024^3 314.2 -> 321.9, FP32 236.9 -> 241.0 Gcell/s.
+  auto prefetch_material = [&](size_t pz) {
     if constexpr (kIdx) {
 #pragma unroll
       for (int r = 0; r < R; ++r) pcb[r] = ldp<uint8_t>(a.m.id, pz >> kLogES, off[r] >> kLogES);
@@ -285,6 +282,20 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
 #pragma unroll
       for (int r = 0; r < R; ++r) pcb[r] = ldp<float>(a.m.cb, pz * kCbMul, off[r] * kCbMul);
     }
+  };
+  // Single-problem tiles load the material ids (or Cb) first: the compiler
+  // moves their registers at the loop's back edge, which needs those loads
+  // complete, and the vector-memory counter completes in issue order, so the
+  // field loads issued after them (and this position's stores) stay in
+  // flight there. Issued last, they made that wait a wait for every
+  // outstanding access (binary16: vmcnt(0) at the back edge): 512^3
+  // binary16 286.7 -> 298.6, 1024^3 314.2 -> 321.9, FP32 236.9 -> 241.0
+  // Gcell/s. The batch kernel's tiles keep the material last (256-instance
+  // batch 214.4 against 210.9 with it first).
+  constexpr bool kMaterialFirst = kSingle;
+  auto prefetch = [&](int pp) {
+    const size_t pz = (size_t)min(pp, nz - 1) * plane_bytes;
+    if constexpr (kMaterialFirst) prefetch_material(pz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
@@ -295,6 +306,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
     }
     phmz = ldp<S>(a.src.c[5], pz, offm);
     phmx = ldp<S>(a.src.c[3], pz, offm);
+    if constexpr (!kMaterialFirst) prefetch_material(pz);
   };
   prefetch(p_begin);
   // With 4-byte fields the loop ends each position with 6R stores. Issuing
@@ -638,9 +650,9 @@ __global__ __launch_bounds__(64 * W, zstream_waves_per_simd<R>()) void zstream_b
     a.n0 = n0;
     int tx, ty, tz;
     tile_coords(a, wi.tile, tx, ty, tz);
-    // branch-guarded stores: with the batch's small tiles and short streams
-    // the fixed-count form measured slower (256-instance batch 199-207
-    // against 213 Gcell/s)
+    // batch form of the tile (kSingle = false): with the batch's small tiles
+    // and short streams the fixed-count stores measured slower (256-instance
+    // batch 199-207 against 213 Gcell/s)
     if (wi.full)
       zstream_tile<B, R, W, kMat, S, true, false>(a, tx, ty, tz);
     else
