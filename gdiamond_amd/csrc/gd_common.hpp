@@ -382,15 +382,27 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // is zero, so every 4- or 8-byte access in the loop would cost a 64-bit add.
 // (For 2-byte elements the plain form measures faster, and so it does for
 // every element size in the FUSED kernel, which passes kScalarBase = false.)
+//
+// Every field, material and snapshot array lives in global device memory, and
+// the accesses say so (gptr): a pointer the compiler cannot prove global (one
+// read from memory, as the batch kernel reads its per-instance arguments)
+// would otherwise be accessed with flat instructions, which also count in the
+// LDS wait counter, so that each wait for an LDS read (the exchanged rows,
+// the Cb table) waited for the prefetched plane as well (256-instance batch
+// with flat accesses 209-213, with global ones 215-219 Gcell/s).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
 template <class S, bool kScalarBase = (sizeof(S) >= 4)>
 __device__ __forceinline__ S ldp(const void* base, size_t plane_off, unsigned off) {
   if constexpr (kScalarBase) asm("" : "+v"(off));
-  return *reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off);
+  return *gptr(reinterpret_cast<const S*>(static_cast<const char*>(base) + plane_off + off));
 }
 template <class S, bool kScalarBase = (sizeof(S) >= 4)>
 __device__ __forceinline__ void stp(void* base, size_t plane_off, unsigned off, S v) {
   if constexpr (kScalarBase) asm("" : "+v"(off));
-  *reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off) = v;
+  *gptr(reinterpret_cast<S*>(static_cast<char*>(base) + plane_off + off)) = v;
 }
 
 // Store through a buffer resource spanning one plane (`plane_bytes` < 2^31

@@ -62,7 +62,7 @@ __device__ __forceinline__ void apply_sources_row(const Sources& s, int gx, int 
     const SourceDesc sd = s.d[q];
     const int k = sd.comp - base;
     if (sd.k != z || sd.j != gy || k < 0 || k > 2) continue;
-    const float v = sd.table[n];
+    const float v = *gptr(sd.table + n);
     const bool m = sd.i == gx;
     c[0] = (m && k == 0) ? c[0] - v * dx : c[0];
     c[1] = (m && k == 1) ? c[1] - v * dx : c[1];
@@ -262,7 +262,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
     (&lds_e[0][0][0][0][0])[q] = 0.0f;
   }
   if constexpr (kIdx) {
-    for (int q = threadIdx.x; q < kMaxMaterials; q += 64 * W) lds_cb[q] = a.m.table[q];
+    for (int q = threadIdx.x; q < kMaxMaterials; q += 64 * W) lds_cb[q] = *gptr(a.m.table + q);
   }
 
   // Level-0 values of the next stream position, loaded one iteration ahead so
@@ -452,7 +452,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
                         ((size_t)(b - 1 + a.d.snap_base) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              if (dft_row(r)) sp[dft_base(r) + zE * dft_zmul] = en[r][c];
+              if (dft_row(r)) gptr(sp)[dft_base(r) + zE * dft_zmul] = en[r][c];
           }
           if (okH && ((a.d.mask >> (3 + c)) & 1u)) {
             const int slot = __builtin_popcount(a.d.mask & ((1u << (3 + c)) - 1u));
@@ -460,7 +460,7 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
                         ((size_t)(b - 1 + a.d.snap_base) * a.d.ncomp + slot) * (size_t)a.d.plane;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-              if (dft_row(r)) sp[dft_base(r) + zH * dft_zmul] = hn[r][c];
+              if (dft_row(r)) gptr(sp)[dft_base(r) + zH * dft_zmul] = hn[r][c];
           }
         }
       }
@@ -553,7 +553,7 @@ __device__ __forceinline__ void uniform_copy(T* dst, const T* src) {
   const int* s = reinterpret_cast<const int*>(src);
   int* d = reinterpret_cast<int*>(dst);
 #pragma unroll
-  for (unsigned q = 0; q < sizeof(T) / 4; ++q) d[q] = __builtin_amdgcn_readfirstlane(s[q]);
+  for (unsigned q = 0; q < sizeof(T) / 4; ++q) d[q] = __builtin_amdgcn_readfirstlane(*gptr(s + q));
 }
 
 // Tile of this workgroup: launch position xcd_tile(blockIdx) in tile order,
