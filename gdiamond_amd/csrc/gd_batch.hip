@@ -222,6 +222,7 @@ gd::ZArgs make_args(const gd_batch* b, int i, int parity, const gd::ZTiling& t) 
   a.run_step0 = 0;
   gd::ztile_counts(t, a.g.nx, a.g.ny, a.g.nz, a.ntx, a.nty, a.ntz);
   a.zc = t.zchunk;
+  a.zsplit = -1;
   return a;
 }
 

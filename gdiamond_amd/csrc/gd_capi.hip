@@ -1059,6 +1059,11 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
       if (tiling->zchunk) t.zchunk = tiling->zchunk;
     }
     zt = gd::ZTiling{t.blt, t.rows, t.waves, t.zchunk};
+    // A monitored z plane gets a z chunk of its own (gd::chunk_range): only
+    // that one-plane layer of tiles then runs the build with the snapshot
+    // code, and the rest of its chunk the lean build.
+    if (c->dft_ncomp && c->dft_axis == 2 && gd::zstream_lean_supported(zt))
+      zt.zsplit = (int)c->dft_index;
     if (c->storage == GD_STORAGE_F64)
       return c->fail(GD_ERR_UNSUPPORTED, "the tiled kernel runs FP32 and 16-bit storage only");
     if (c->nx * c->ny * 4 >= (size_t(1) << 31))

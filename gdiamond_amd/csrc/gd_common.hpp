@@ -438,9 +438,69 @@ hipError_t launch_fused_step(Storage st, const Fields& in, const Fields& out, co
                              const Material& m, MatKind mk, const Grid& g, const Sources& s,
                              const Dft& d, int n, int run_step, hipStream_t stream);
 
+// zsplit >= 0: the z chunk holding plane zsplit is cut into the part below
+// it, the one-plane chunk [zsplit, zsplit + 1) and the part above it (see
+// chunk_range); -1: uniform chunks of zchunk planes.
 struct ZTiling {
   int blt, rows, waves, zchunk;
+  int zsplit = -1;
 };
+
+// Planes [lo, hi) of z chunk tz: chunks of zc planes, except that with
+// zsplit >= 0 the chunk holding plane zsplit is cut into [start, zsplit)
+// (when not empty), [zsplit, zsplit + 1) and [zsplit + 1, end) (when not
+// empty). A TILED launch with a monitored z plane cuts there, so that only
+// the tiles of the one-plane chunk run the build with the snapshot code, and
+// every other tile of that layer runs the lean build.
+__host__ __device__ inline void chunk_range(int zc, int nz, int zsplit, int tz, int& lo, int& hi) {
+  if (zsplit < 0) {
+    lo = tz * zc;
+    hi = lo + zc < nz ? lo + zc : nz;
+    return;
+  }
+  const int c = zsplit / zc;
+  const int end = (c + 1) * zc < nz ? (c + 1) * zc : nz;
+  const int below = zsplit > c * zc ? 1 : 0;
+  const int above = zsplit + 1 < end ? 1 : 0;
+  int k = tz - c;
+  if (k < 0) {
+    lo = tz * zc;
+    hi = lo + zc;
+    return;
+  }
+  if (below) {
+    if (k == 0) {
+      lo = c * zc;
+      hi = zsplit;
+      return;
+    }
+    --k;
+  }
+  if (k == 0) {
+    lo = zsplit;
+    hi = zsplit + 1;
+    return;
+  }
+  --k;
+  if (above) {
+    if (k == 0) {
+      lo = zsplit + 1;
+      hi = end;
+      return;
+    }
+    --k;
+  }
+  lo = (c + 1 + k) * zc;
+  hi = lo + zc < nz ? lo + zc : nz;
+}
+// Number of z chunks of chunk_range.
+__host__ __device__ inline int chunk_count(int zc, int nz, int zsplit) {
+  const int n = (nz + zc - 1) / zc;
+  if (zsplit < 0) return n;
+  const int c = zsplit / zc;
+  const int end = (c + 1) * zc < nz ? (c + 1) * zc : nz;
+  return n + (zsplit > c * zc ? 1 : 0) + (zsplit + 1 < end ? 1 : 0);
+}
 
 // Arguments of one TILED launch over one problem: source and destination
 // field copies, decomposition into ntx*nty*ntz tiles of zc planes.
@@ -458,6 +518,7 @@ struct ZArgs {
   int zc;         // planes owned per workgroup
   int scw;        // tile order: 0 = x fastest along whole rows; w > 0 = columns
                   // of w tiles, x fastest within a column, when ntx > w
+  int zsplit;     // chunk_range's cut plane, -1 for uniform chunks
 };
 
 // Tile decomposition of an nx*ny*nz problem under a tiling.
@@ -466,7 +527,7 @@ inline void ztile_counts(const ZTiling& t, int nx, int ny, int nz, int& ntx, int
   const int oy = t.rows * t.waves - 2 * t.blt;
   ntx = (nx + ox - 1) / ox;
   nty = (ny + oy - 1) / oy;
-  ntz = (nz + t.zchunk - 1) / t.zchunk;
+  ntz = chunk_count(t.zchunk, nz, t.zsplit);
 }
 
 // Batched persistent launch: every workgroup takes (instance, tile) work items

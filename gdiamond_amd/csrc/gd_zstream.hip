@@ -196,8 +196,8 @@ __device__ __forceinline__ void zstream_tile(const ZArgs& a, const int tx, const
   // wave-uniform: does any lane of this wave hold a source column?
   const bool wave_src = kX && __builtin_amdgcn_ballot_w64(any_src) != 0;
 
-  const int z_lo = tz * a.zc;
-  const int z_hi = min(nz, z_lo + a.zc);
+  int z_lo, z_hi;
+  chunk_range(a.zc, nz, a.zsplit, tz, z_lo, z_hi);
   const int p_begin = max(0, z_lo - B);
   const int p_end = z_hi - 1 + B;
 
@@ -854,7 +854,8 @@ bool tile_needs_full(const ZTiling& t, const Grid& g, const SourceDesc* src, int
   const int B = t.blt;
   const int ox = 64 - 2 * B, oy = t.rows * t.waves - 2 * B;
   const int x0 = tx * ox - B, y0 = ty * oy - B;
-  const int z_lo = tz * t.zchunk, z_hi = std::min(g.nz, z_lo + t.zchunk);
+  int z_lo, z_hi;
+  chunk_range(t.zchunk, g.nz, t.zsplit, tz, z_lo, z_hi);
   for (int q = 0; q < nsrc; ++q)
     if (src[q].i >= x0 && src[q].i < x0 + 64 && src[q].j >= y0 &&
         src[q].j < y0 + t.rows * t.waves && src[q].k >= z_lo - 2 * B &&
@@ -1008,6 +1009,7 @@ hipError_t launch_zstream(const ZTiling& t, Storage st, const Fields& src, const
   a.run_step0 = run_step0;
   ztile_counts(t, g.nx, g.ny, g.nz, a.ntx, a.nty, a.ntz);
   a.zc = t.zchunk;
+  a.zsplit = t.zsplit;
   // FP32 on rows of more than 10 tiles: columns 10 tiles wide keep the tiles
   // an XCD runs together in a compact block that shares halo rows in L2
   // (1024^3: ~200 instead of ~190 Gcell/s); 16-bit launches measured no gain
