@@ -68,6 +68,7 @@ struct gd_ctx {
   double* dft_w = nullptr;
   size_t dft_w_cap = 0;
   float* dft_snap = nullptr;  // TILED: [level][slot][plane] plane values
+  int dft_snap_levels = 0;    // levels dft_snap holds (set when allocated)
   std::vector<double> dft_w_host;
 
   // a run enqueued by gd_run_async that gd_sync (or the next call on the
@@ -921,6 +922,7 @@ int gd_dft_disable(gd_ctx* c) {
   if (c->dft_snap) (void)hipFree(c->dft_snap);
   c->dft_re = c->dft_im = nullptr;
   c->dft_snap = nullptr;
+  c->dft_snap_levels = 0;
   c->dft_ncomp = 0;
   c->dft_f.clear();
   c->dft_plane = 0;
@@ -1016,7 +1018,7 @@ int gd_device_bytes(const gd_ctx* c, size_t* bytes) {
     if (c->coeff[a]) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
   if (c->cb) b += c->N * (c->storage == GD_STORAGE_F64 ? 8 : 4);
   if (c->mat_id) b += c->N + GD_MAX_MATERIALS * sizeof(float);
-  if (c->dft_snap) b += (size_t)gd::kMaxBlt * c->dft_ncomp * c->dft_plane * sizeof(float);
+  if (c->dft_snap) b += (size_t)c->dft_snap_levels * c->dft_ncomp * c->dft_plane * sizeof(float);
   for (auto& s : c->src) b += s.len * sizeof(float);
   b += 2 * (size_t)c->dft_ncomp * c->dft_f.size() * c->dft_plane * sizeof(double);
   b += c->dft_w_cap * sizeof(double);
@@ -1074,9 +1076,18 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
     int rc = alloc_fields(c, 1);
     if (rc != GD_OK) return rc;
     if (c->dft_ncomp && !c->dft_snap) {
-      const size_t n = (size_t)gd::kMaxBlt * c->dft_ncomp * c->dft_plane;
+      // as many levels as fit in 256 MiB, between kMaxBlt and 32: each fold
+      // of the snapshots into the FP64 sums is a kernel and a dependency
+      // between two launches, and reads and writes the sums once
+      const size_t per_level = (size_t)c->dft_ncomp * c->dft_plane * sizeof(float);
+      const size_t fit = (size_t(256) << 20) / std::max<size_t>(1, per_level);
+      c->dft_snap_levels = (int)std::min<size_t>(32, std::max<size_t>(gd::kMaxBlt, fit));
+      const size_t n = (size_t)c->dft_snap_levels * c->dft_ncomp * c->dft_plane;
       hipError_t e = hipMalloc(&c->dft_snap, n * sizeof(float));
-      if (e != hipSuccess) return c->hip_fail(e, "hipMalloc(dft snapshot)");
+      if (e != hipSuccess) {
+        c->dft_snap_levels = 0;
+        return c->hip_fail(e, "hipMalloc(dft snapshot)");
+      }
     }
   }
 
@@ -1248,8 +1259,9 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
       sp.blt = t.blt;
       return GD_OK;
     };
-    // the snapshot buffer holds kMaxBlt levels: the DFT fold runs once per
-    // pair of launches (B = 3, 4), which saves a kernel and a dependency
+    // the snapshot buffer holds dft_snap_levels levels (32 for the 512^3 and
+    // 1024^3 workloads): the DFT fold runs once per 8 to 10 launches, not
+    // after every one
     size_t r = 0;
     int snap_levels = 0;  // levels in the snapshot buffer not folded yet
     size_t snap_r0 = 0;   // run step of the first of them
@@ -1295,7 +1307,7 @@ int gd_run_async(gd_ctx* c, int variant, size_t nsteps, const gd_tiling* tiling)
         snap_levels += t.blt;
         const size_t rn = r + (size_t)t.blt;
         const size_t next = std::min<size_t>((size_t)zt.blt, nsteps - rn);
-        if (rn >= nsteps || snap_levels + (int)next > gd::kMaxBlt) {
+        if (rn >= nsteps || snap_levels + (int)next > c->dft_snap_levels) {
           GD_HIP(c, gd::launch_dft_snapshot(d, snap_levels, (int)snap_r0, c->stream));
           snap_levels = 0;
           snap_r0 = rn;
