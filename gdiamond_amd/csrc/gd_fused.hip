@@ -166,8 +166,17 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
   S pe[R][3], ph[R][3], p_hzb, p_hxb, p_exa, p_eza, p_hxa, p_hya, p_hza;
   MT pm[R], pma;
   auto raw = [&](int c, size_t pz, unsigned q) { return ldp<S, false>(in.c[c], pz * ES, q * ES); };
+  // The material first: the top of the next iteration looks Cb up in LDS
+  // from these ids before it needs every field value, and the vector-memory
+  // counter completes in issue order (512^3 binary16 131.3 -> 136.6 Gcell/s,
+  // FP32 unchanged).
   auto prefetch = [&](int kk) {
     const size_t pz = (size_t)kk * plane_cells;
+    if constexpr (!kFull) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) pm[r] = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell[r] * MS);
+      pma = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell_a * MS);
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -182,11 +191,6 @@ __global__ __launch_bounds__(64 * kFuW) void fused_step_kernel(Fields in, Fields
     p_hxa = raw(3, pz, cell_a);
     p_hya = raw(4, pz, cell_a);
     p_hza = raw(5, pz, cell_a);
-    if constexpr (!kFull) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) pm[r] = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell[r] * MS);
-      pma = ldp<MT, false>(kIdx ? (const void*)m.id : m.cb, pz * MS, cell_a * MS);
-    }
   };
   prefetch(k0);
 
