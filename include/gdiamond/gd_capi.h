@@ -366,7 +366,9 @@ int gd_batch_set_class_tiling(gd_batch* b, int cls, const gd_tiling* t);
 int gd_batch_get_class_tiling(const gd_batch* b, int cls, gd_tiling* t);
 /* Autotuner: for each size class, times every supported tiling (blt 1..4,
  * rows x waves in {2,4}x{4,8}, zchunk in {16,32,64,nz}) as persistent launches
- * over that class's instances and keeps the fastest per cell-update. Fields
+ * over that class's instances and keeps, among the tilings within 3 % of the
+ * fastest per cell-update, the one with the most levels per launch, then the
+ * tallest tile, then the fastest. Fields
  * and the step counter are left unchanged (the timing launches always read
  * the current copy). `reps` timed launches per candidate (>= 1). */
 int gd_batch_autotune(gd_batch* b, int reps);
